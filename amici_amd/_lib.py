@@ -1,0 +1,91 @@
+"""ctypes binding of the C ABI declared in include/amx.h.
+
+The product path has no CPU fallback: loading a model module that is missing
+or was built without its HIP kernels raises immediately.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from pathlib import Path
+
+AMX_ABI_VERSION = 1
+AMX_NSTATS = 16
+SENSI_NONE, SENSI_FORWARD, SENSI_ADJOINT = 0, 1, 2
+
+SYMBOLS = ("amx_model_info", "amx_workspace_bytes", "amx_run_batch", "amx_run_sequential",
+           "amx_status_string")
+
+
+class SolverOpts(ctypes.Structure):
+    _fields_ = [
+        ("rtol", ctypes.c_double), ("atol", ctypes.c_double),
+        ("rtol_fsa", ctypes.c_double), ("atol_fsa", ctypes.c_double),
+        ("rtolB", ctypes.c_double), ("atolB", ctypes.c_double),
+        ("quad_rtol", ctypes.c_double), ("quad_atol", ctypes.c_double),
+        ("maxsteps", ctypes.c_int), ("maxstepsB", ctypes.c_int),
+        ("sensi_meth", ctypes.c_int), ("max_traj", ctypes.c_int),
+    ]
+
+
+class BatchIn(ctypes.Structure):
+    _fields_ = [
+        ("nb", ctypes.c_int), ("nt", ctypes.c_int), ("ts_stride", ctypes.c_int),
+        ("theta", ctypes.c_void_p), ("kfix", ctypes.c_void_p), ("ts", ctypes.c_void_p),
+        ("meas", ctypes.c_void_p), ("sigma", ctypes.c_void_p),
+    ]
+
+
+class BatchOut(ctypes.Structure):
+    _fields_ = [
+        ("x", ctypes.c_void_p), ("y", ctypes.c_void_p), ("sx", ctypes.c_void_p),
+        ("llh", ctypes.c_void_p), ("sllh", ctypes.c_void_p),
+        ("status", ctypes.c_void_p), ("stats", ctypes.c_void_p),
+    ]
+
+
+class ModelLib:
+    """A loaded model module ``libamx_<name>.so``."""
+
+    def __init__(self, path: str | Path):
+        path = Path(path)
+        if not path.exists():
+            raise FileNotFoundError(
+                f"HIP model module {path} is missing; build it with __graft_entry__.build() "
+                "or amici_amd.codegen.compile_model")
+        self.path = path
+        self.lib = ctypes.CDLL(str(path))
+        for s in SYMBOLS:
+            if not hasattr(self.lib, s):
+                raise RuntimeError(f"{path} does not export {s}")
+        L = self.lib
+        L.amx_model_info.argtypes = [ctypes.POINTER(ctypes.c_int)]
+        L.amx_model_info.restype = ctypes.c_int
+        L.amx_workspace_bytes.argtypes = [ctypes.POINTER(SolverOpts), ctypes.c_int, ctypes.c_int]
+        L.amx_workspace_bytes.restype = ctypes.c_size_t
+        for fn in (L.amx_run_batch, L.amx_run_sequential):
+            fn.argtypes = [ctypes.POINTER(SolverOpts), ctypes.POINTER(BatchIn),
+                           ctypes.POINTER(BatchOut), ctypes.c_void_p, ctypes.c_size_t,
+                           ctypes.c_void_p]
+            fn.restype = ctypes.c_int
+        L.amx_status_string.argtypes = [ctypes.c_int]
+        L.amx_status_string.restype = ctypes.c_char_p
+        dims = (ctypes.c_int * 8)()
+        if L.amx_model_info(dims) != 0:
+            raise RuntimeError("amx_model_info failed")
+        self.nx, self.nr, self.np, self.nk, self.ny, self.team, self.noise, abi = list(dims)
+        if abi != AMX_ABI_VERSION:
+            raise RuntimeError(f"{path}: ABI version {abi} != {AMX_ABI_VERSION}")
+
+    def status_string(self, s: int) -> str:
+        return self.lib.amx_status_string(int(s)).decode()
+
+    def workspace_bytes(self, opts: SolverOpts, nb: int, nt: int) -> int:
+        return int(self.lib.amx_workspace_bytes(ctypes.byref(opts), int(nb), int(nt)))
+
+
+def make_opts(rtol=1e-8, atol=1e-16, sensi_meth=SENSI_NONE, maxsteps=10000, rtol_fsa=math.nan,
+              atol_fsa=math.nan, rtolB=math.nan, atolB=math.nan, quad_rtol=1e-8,
+              quad_atol=1e-12, maxstepsB=0, max_traj=0) -> SolverOpts:
+    return SolverOpts(rtol, atol, rtol_fsa, atol_fsa, rtolB, atolB, quad_rtol, quad_atol,
+                      int(maxsteps), int(maxstepsB), int(sensi_meth), int(max_traj))

@@ -1,0 +1,977 @@
+// Variable-order BDF integrator for one ODE instance per team of lanes
+// (gfx950, FP64).  Each team lane i < NX owns state component i.  The step,
+// order and Newton heuristics follow SUNDIALS CVODES 7.1.1 as AMICI drives it
+// (BDF, Newton, dense direct solver, tstop at every output time):
+//   cvStep                       ThirdParty/sundials/src/cvodes/cvodes.c:5800-6094
+//   cvAdjustParams/Order/BDF     :6112-6423      cvRescale       :6425
+//   cvPredict                    :6507           cvSetBDF/TqBDF  :6788-6880
+//   cvNls + Newton               :6894, sunnonlinsol_newton.c, cvodes_nls.c:244-353
+//   cvLsSetup / cvLsSolve        cvodes_ls.c:1611-1853 (J reuse, 2/(1+gamrat) scaling)
+//   cvHandleNFlag / cvRestore    cvodes.c:7309-7400
+//   cvDoErrorTest                :7461           cvCompleteStep  :7634
+//   cvPrepareNextStep/SetEta     :7747-7830      cvComputeEtaqm1/qp1, cvChooseEta :7836-8010
+//   cvHin / cvUpperBoundH0 / cvYddNorm :5436-5798 CVode tstop handling :3100-3620
+//   constants                    cvodes_impl.h:57-204, cvodes_ls_impl.h:40-42
+//   quadratures (cvQuadNls)      cvodes.c:7089;  staggered FSA (cvStgrNls) :7167,
+//                                cvodes_nls_stg.c (residual / lsolve / conv test)
+// Dense LU with partial pivoting replaces SUNDlsMat_denseGETRF/GETRS
+// (sundials_dense.c:82-148): rows stay in the registers of their lane and the
+// pivot order is kept as a permutation instead of swapping rows.
+// Not modelled: BDF stability-limit detection (cvBDFStab), hmin/hmax limits.
+#pragma once
+#include "amx_team.h"
+
+namespace amx {
+
+constexpr int QMAX = 5;
+constexpr int LMAX = QMAX + 1;
+constexpr double UROUND = 2.220446049250313e-16;
+
+// CVODES tuning constants.
+constexpr double ETA_MAX_FX = 1.5, ETA_MAX_FS = 10000.0, ETA_MAX_ES = 10.0,
+                 ETA_MAX_GS = 10.0, ETA_MIN = 0.1, ETA_MAX_EF = 0.2,
+                 ETA_MIN_EF = 0.1, ETA_CF = 0.25;
+constexpr int SMALL_NST = 10, SMALL_NEF = 2, MXNEF1 = 3, LONG_WAIT = 10,
+              MXNCF = 10, MXNEF = 7, NLS_MAXCOR = 3, MSBP = 20, MSBJ = 51;
+constexpr double ADDON = 1e-6, BIAS1 = 6.0, BIAS2 = 6.0, BIAS3 = 10.0,
+                 DGMAX_LSETUP = 0.3, DGMAX_JBAD = 0.2, CRDOWN = 0.3,
+                 RDIV = 2.0, CORTES = 0.1, FUZZ_FACTOR = 100.0,
+                 HLB_FACTOR = 100.0, HUB_FACTOR = 0.1, H_BIAS = 0.5;
+
+// AMICI status codes (include/amici/defines.h:66-99) plus one of this library.
+enum : int {
+  AMX_SUCCESS = 0,
+  AMX_TOO_MUCH_WORK = -1,
+  AMX_TOO_MUCH_ACC = -2,
+  AMX_ERR_FAILURE = -3,
+  AMX_CONV_FAILURE = -4,
+  AMX_LSETUP_FAIL = -6,
+  AMX_RHSFUNC_FAIL = -8,
+  AMX_ILL_INPUT = -22,
+  AMX_TOO_CLOSE = -27,
+  AMX_NOT_RUN = -1001,
+  AMX_TRAJ_FULL = -1010,  // forward-trajectory buffer of an instance exhausted
+};
+
+// internal flags
+enum : int { FIRST_CALL = 0, PREV_CONV_FAIL = 1, PREV_ERR_FAIL = 2 };
+enum : int { DO_ERROR_TEST = 10, PREDICT_AGAIN = 11, TRY_AGAIN = 12 };
+enum : int { NLS_OK = 0, NLS_RECOVER = 1 };
+enum : int { NO_FAILURES = 0, FAIL_BAD_J = 1, FAIL_OTHER = 2 };
+
+__device__ __forceinline__ double powr(double b, double e) { return pow(b, e); }
+
+// Hides a value from the optimizer so that a select between array elements is
+// not rewritten into a load through a selected address (which would force the
+// whole register-resident integrator state into scratch memory).
+__device__ __forceinline__ double opaque(double v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
+// a[i] for a runtime index i, without dynamic indexing.
+template <int N>
+__device__ __forceinline__ double sel(const double (&a)[N], int i) {
+  double r = opaque(a[0]);
+#pragma unroll
+  for (int k = 1; k < N; ++k) {
+    const double v = opaque(a[k]);
+    r = (i == k) ? v : r;
+  }
+  return r;
+}
+
+// Row-per-lane dense LU of an NX x NX matrix held by the first NX team lanes.
+template <int NX, int T>
+struct LURows {
+  double a[NX];
+  double dinv;   // 1 / pivot of the row this lane holds
+  int pos;       // elimination step at which this lane's row was the pivot
+  int src;       // lane that holds pivot row number `lane`
+  int perm[NX];  // perm[k] = lane of the k-th pivot row (team-uniform)
+
+  __device__ __forceinline__ bool factor(const Team<T>& tm) {
+    pos = (tm.lane < NX) ? -1 : NX;
+    src = 0;
+    dinv = 0.0;
+#pragma unroll
+    for (int k = 0; k < NX; ++k) {
+      double v = (pos < 0) ? fabs(a[k]) : -1.0;
+      int p;
+      const double mv = tm.argmax(v, p);
+      if (!(mv > 0.0)) return false;  // zero pivot: recoverable setup failure
+      perm[k] = p;
+      if (tm.lane == k) src = p;
+      double prow[NX];
+#pragma unroll
+      for (int j = k; j < NX; ++j) prow[j] = tm.bcast(a[j], p);
+      const double inv = 1.0 / prow[k];
+      if (tm.lane == p) { pos = k; dinv = inv; }
+      if (pos < 0) {
+        const double l = a[k] * inv;
+        a[k] = l;
+#pragma unroll
+        for (int j = k + 1; j < NX; ++j) a[j] = fma(-prow[j], l, a[j]);
+      }
+    }
+    return true;
+  }
+
+  // Solves (LU) x = b; b is this lane's component of the right-hand side.
+  __device__ __forceinline__ double solve(double b, const Team<T>& tm) const {
+#pragma unroll
+    for (int k = 0; k < NX; ++k) {
+      const double yk = tm.bcast(b, perm[k]);
+      if (pos > k) b = fma(-a[k], yk, b);
+    }
+#pragma unroll
+    for (int k = NX - 1; k >= 0; --k) {
+      const double xk = tm.bcast(b * dinv, perm[k]);
+      if (pos == k) b = xk;
+      else if (pos < k) b = fma(-a[k], xk, b);
+    }
+    return tm.gather(b, src);
+  }
+};
+
+struct SolverParams {
+  double rtol, atol;
+  double rtolQ, atolQ;
+  double rtolS, atolS;
+  int maxsteps;
+  int errconQ;
+};
+
+// BDF integrator over a Problem P.  P provides:
+//   NX, T, NQ (quadratures, 0 if none), NS (forward sensitivities, 0 if none)
+//   double rhs(double t, double y)                    team-collective, returns f_lane
+//   bool   quad_rhs(double t, double y, double* fq)   fq[EPQ] (if NQ > 0)
+//   void   jac(double t, double y)                    assembles J into its LDS
+//   void   build_M(double gamma, double* a)           a[j] = (I - gamma*J)[lane][j]
+//   sensitivity hooks (if NS > 0): see sens_* below
+template <class P>
+struct BDF {
+  static constexpr int NX = P::NX;
+  static constexpr int T = P::T;
+  static constexpr int NQ = P::NQ;
+  static constexpr int EPQ = (NQ > 0) ? (NQ + T - 1) / T : 1;
+  static constexpr bool QUAD = NQ > 0;
+  static constexpr bool SENS = P::NS > 0;
+
+  P pb;
+  Team<T> tm;
+  SolverParams sp;
+  bool active;  // lane < NX
+
+  // per-lane components
+  double zn[LMAX];
+  double ewt, acor, y, ftemp, tempv;
+  double znQ[LMAX][EPQ], ewtQ[EPQ], acorQ[EPQ], yQ[EPQ];
+  LURows<NX, T> lu;
+
+  // team-uniform scalars
+  double tn, h, hprime, hscale, eta, etamax, etaq, etaqm1, etaqp1, gamma,
+      gammap, gamrat, rl1, crate, crateS, delp, acnrm, acnrmS, saved_tq5, hu,
+      tolsf, tstop, dsm_last, dsm_test;
+  double tau[LMAX + 1], l[LMAX], tq[6];
+  int q, qprime, L, qwait, nst, nstlp, nstlj, nscon, convfail;
+  bool jcur, tstopset;
+  // counters
+  int nfe, nsetups, nje, nni, netf, ncfn, nfQe, nniS;
+
+  __device__ __forceinline__ BDF(const P& p, const SolverParams& s) : pb(p), sp(s) {
+    active = tm.lane < NX;
+  }
+
+  // ---------------- vector helpers ----------------
+  __device__ __forceinline__ double wrms(double v, double w) const {
+    const double e = active ? v * w : 0.0;
+    return sqrt(tm.sum(e * e) / NX);
+  }
+  __device__ __forceinline__ double wrmsQ(const double (&v)[EPQ], const double (&w)[EPQ]) const {
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < EPQ; ++k) {
+      const int g = tm.lane + k * T;
+      const double e = (g < NQ) ? v[k] * w[k] : 0.0;
+      acc += e * e;
+    }
+    return sqrt(tm.sum(acc) / (NQ > 0 ? NQ : 1));
+  }
+  __device__ __forceinline__ void set_ewt() { ewt = active ? 1.0 / (sp.rtol * fabs(zn[0]) + sp.atol) : 0.0; }
+  __device__ __forceinline__ void set_ewtQ() {
+#pragma unroll
+    for (int k = 0; k < EPQ; ++k) {
+      const int g = tm.lane + k * T;
+      ewtQ[k] = (g < NQ) ? 1.0 / (sp.rtolQ * fabs(znQ[0][k]) + sp.atolQ) : 0.0;
+    }
+  }
+  __device__ __forceinline__ bool quad_errcon() const { return QUAD && sp.errconQ; }
+
+  // ---------------- initialisation ----------------
+  // CVodeInit state (cvodes.c:677-886) for y0 on this lane, quadratures zero.
+  __device__ __forceinline__ void init(double t0, double y0, const double* q0) {
+    tn = t0;
+#pragma unroll
+    for (int j = 0; j < LMAX; ++j) zn[j] = 0.0;
+    zn[0] = active ? y0 : 0.0;
+    if constexpr (QUAD) {
+#pragma unroll
+      for (int j = 0; j < LMAX; ++j)
+#pragma unroll
+        for (int k = 0; k < EPQ; ++k) znQ[j][k] = 0.0;
+#pragma unroll
+      for (int k = 0; k < EPQ; ++k) znQ[0][k] = q0 ? q0[k] : 0.0;
+    }
+    q = 1; L = 2; qwait = L; etamax = ETA_MAX_FS;
+    hu = 0.0; tolsf = 1.0; h = 0.0; hprime = 0.0; hscale = 0.0; eta = 1.0;
+    nst = 0; nstlp = 0; nstlj = 0; nscon = 0; saved_tq5 = 0.0;
+    gammap = 0.0; gamma = 0.0; gamrat = 1.0; crate = 1.0; crateS = 1.0;
+    delp = 0.0; acnrm = 0.0; acnrmS = 0.0; convfail = NO_FAILURES;
+    jcur = false; tstopset = false; etaq = etaqm1 = etaqp1 = 0.0;
+#pragma unroll
+    for (int j = 0; j <= LMAX; ++j) tau[j] = 0.0;
+#pragma unroll
+    for (int j = 0; j < LMAX; ++j) l[j] = 0.0;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) tq[j] = 0.0;
+    nfe = nsetups = nje = nni = netf = ncfn = nfQe = nniS = 0;
+    dsm_last = 0.0;
+  }
+
+  // ---------------- Nordsieck array manipulation ----------------
+  template <class F>
+  __device__ __forceinline__ void for_zn(F&& f) {  // f(j, double& znj_state, double (&znQj)[EPQ])
+#pragma unroll
+    for (int j = 0; j < LMAX; ++j) f(j, zn[j], znQ[j]);
+  }
+
+  __device__ __forceinline__ void predict() {
+    tn += h;
+    if (tstopset && (tn - tstop) * h > 0.0) tn = tstop;
+#pragma unroll
+    for (int k = 1; k <= QMAX; ++k)
+#pragma unroll
+      for (int j = QMAX; j >= k; --j)
+        if (k <= q && j <= q) {
+          zn[j - 1] += zn[j];
+          if constexpr (QUAD)
+#pragma unroll
+            for (int e = 0; e < EPQ; ++e) znQ[j - 1][e] += znQ[j][e];
+        }
+    if constexpr (SENS) pb.sens_predict(q);
+  }
+
+  __device__ __forceinline__ void restore(double saved_t) {
+    tn = saved_t;
+#pragma unroll
+    for (int k = 1; k <= QMAX; ++k)
+#pragma unroll
+      for (int j = QMAX; j >= k; --j)
+        if (k <= q && j <= q) {
+          zn[j - 1] -= zn[j];
+          if constexpr (QUAD)
+#pragma unroll
+            for (int e = 0; e < EPQ; ++e) znQ[j - 1][e] -= znQ[j][e];
+        }
+    if constexpr (SENS) pb.sens_restore(q);
+  }
+
+  __device__ __forceinline__ void rescale() {
+    double c = eta;
+#pragma unroll
+    for (int j = 1; j <= QMAX; ++j) {
+      if (j <= q) {
+        zn[j] *= c;
+        if constexpr (QUAD)
+#pragma unroll
+          for (int e = 0; e < EPQ; ++e) znQ[j][e] *= c;
+      }
+      c *= eta;
+    }
+    if constexpr (SENS) pb.sens_rescale(q, eta);
+    h = hscale * eta;
+    hscale = h;
+    nscon = 0;
+  }
+
+  // Adds coef[j] * zn[src] to zn[j] for 2 <= j <= jmax.
+  __device__ __forceinline__ void increase_bdf() {
+    double ll[LMAX];
+#pragma unroll
+    for (int i = 0; i < LMAX; ++i) ll[i] = 0.0;
+    ll[2] = 1.0;
+    double alpha1 = 1.0, prod = 1.0, xiold = 1.0, alpha0 = -1.0, hsum = hscale;
+#pragma unroll
+    for (int j = 1; j < QMAX; ++j) {
+      if (j < q) {
+        hsum += tau[j + 1];
+        const double xi = hsum / hscale;
+        prod *= xi;
+        alpha0 -= 1.0 / (j + 1);
+        alpha1 += 1.0 / xi;
+#pragma unroll
+        for (int i = j + 2; i >= 2; --i)
+          if (i < LMAX) ll[i] = ll[i] * xiold + ll[i - 1];
+        xiold = xi;
+      }
+    }
+    const double A1 = (-alpha0 - alpha1) / prod;
+    // zn[L] = A1 * zn[QMAX] (saved acor);  zn[j] += l[j] * zn[L], j = 2..q
+    double znL = A1 * zn[QMAX];
+#pragma unroll
+    for (int j = 2; j <= QMAX; ++j)
+      if (j <= q) zn[j] = fma(ll[j], znL, zn[j]);
+#pragma unroll
+    for (int j = 2; j <= QMAX; ++j) zn[j] = (j == q + 1) ? opaque(znL) : zn[j];
+    if constexpr (QUAD) {
+#pragma unroll
+      for (int e = 0; e < EPQ; ++e) {
+        double zQL = A1 * znQ[QMAX][e];
+#pragma unroll
+        for (int j = 2; j <= QMAX; ++j)
+          if (j <= q) znQ[j][e] = fma(ll[j], zQL, znQ[j][e]);
+#pragma unroll
+        for (int j = 2; j <= QMAX; ++j) znQ[j][e] = (j == q + 1) ? opaque(zQL) : znQ[j][e];
+      }
+    }
+    if constexpr (SENS) pb.sens_increase(q, A1, ll);
+  }
+
+  __device__ __forceinline__ void decrease_bdf() {
+    double ll[LMAX];
+#pragma unroll
+    for (int i = 0; i < LMAX; ++i) ll[i] = 0.0;
+    ll[2] = 1.0;
+    double hsum = 0.0;
+#pragma unroll
+    for (int j = 1; j <= QMAX - 2; ++j) {
+      if (j <= q - 2) {
+        hsum += tau[j];
+        const double xi = hsum / hscale;
+#pragma unroll
+        for (int i = j + 2; i >= 2; --i) ll[i] = ll[i] * xi + ll[i - 1];
+      }
+    }
+    if (q > 2) {
+      const double znq = sel(zn, q);
+#pragma unroll
+      for (int j = 2; j < QMAX; ++j)
+        if (j < q) zn[j] = fma(-ll[j], znq, zn[j]);
+      if constexpr (QUAD) {
+#pragma unroll
+        for (int e = 0; e < EPQ; ++e) {
+          double zq = opaque(znQ[0][e]);
+#pragma unroll
+          for (int j = 1; j < LMAX; ++j) {
+            const double v = opaque(znQ[j][e]);
+            zq = (j == q) ? v : zq;
+          }
+#pragma unroll
+          for (int j = 2; j < QMAX; ++j)
+            if (j < q) znQ[j][e] = fma(-ll[j], zq, znQ[j][e]);
+        }
+      }
+      if constexpr (SENS) pb.sens_decrease(q, ll);
+    }
+  }
+
+  __device__ __forceinline__ void adjust_order(int deltaq) {
+    if (q == 2 && deltaq != 1) return;
+    if (deltaq == 1) increase_bdf();
+    else if (deltaq == -1) decrease_bdf();
+  }
+
+  __device__ __forceinline__ void adjust_params() {
+    if (qprime != q) {
+      adjust_order(qprime - q);
+      q = qprime;
+      L = q + 1;
+      qwait = L;
+    }
+    rescale();
+  }
+
+  // ---------------- method coefficients ----------------
+  __device__ __forceinline__ void set_bdf() {
+    double xi_inv = 1.0, xistar_inv = 1.0, alpha0 = -1.0, alpha0_hat = -1.0;
+    double hsum = h;
+#pragma unroll
+    for (int i = 0; i < LMAX; ++i) l[i] = 0.0;
+    l[0] = l[1] = 1.0;
+    if (q > 1) {
+#pragma unroll
+      for (int j = 2; j < QMAX; ++j) {
+        if (j < q) {
+          hsum += tau[j - 1];
+          xi_inv = h / hsum;
+          alpha0 -= 1.0 / j;
+#pragma unroll
+          for (int i = j; i >= 1; --i) l[i] += l[i - 1] * xi_inv;
+        }
+      }
+      alpha0 -= 1.0 / q;
+      xistar_inv = -l[1] - alpha0;
+      hsum += sel(tau, q - 1);
+      xi_inv = h / hsum;
+      alpha0_hat = -l[1] - xi_inv;
+#pragma unroll
+      for (int i = QMAX; i >= 1; --i)
+        if (i <= q) l[i] += l[i - 1] * xistar_inv;
+    }
+    // cvSetTqBDF
+    const double A1 = 1.0 - alpha0_hat + alpha0;
+    const double A2 = 1.0 + q * A1;
+    tq[2] = fabs(A1 / (alpha0 * A2));
+    tq[5] = fabs(A2 * xistar_inv / (sel(l, q) * xi_inv));
+    if (qwait == 1) {
+      if (q > 1) {
+        const double C = xistar_inv / sel(l, q);
+        const double A3 = alpha0 + 1.0 / q;
+        const double A4 = alpha0_hat + xi_inv;
+        const double Cpinv = (1.0 - A4 + A3) / A3;
+        tq[1] = fabs(C * Cpinv);
+      } else {
+        tq[1] = 1.0;
+      }
+      hsum += sel(tau, q);
+      xi_inv = h / hsum;
+      const double A5 = alpha0 - 1.0 / (q + 1);
+      const double A6 = alpha0_hat - xi_inv;
+      const double Cppinv = (1.0 - A6 + A5) / A2;
+      tq[3] = fabs(Cppinv / (xi_inv * (q + 2) * A5));
+    }
+    tq[4] = CORTES / tq[2];
+  }
+
+  __device__ __forceinline__ void set_coeffs() {
+    set_bdf();
+    rl1 = 1.0 / l[1];
+    gamma = h * rl1;
+    if (nst == 0) gammap = gamma;
+    gamrat = (nst > 0) ? gamma / gammap : 1.0;
+  }
+
+  // ---------------- linear system ----------------
+  // cvLsSetup + cvNlsLSetup; returns NLS_OK or NLS_RECOVER (singular).
+  __device__ __forceinline__ int lsetup(int cf) {
+    const double dgamma = fabs(gamma / gammap - 1.0);
+    const bool jbad = (nst == 0) || (nst >= nstlj + MSBJ) ||
+                      (cf == FAIL_BAD_J && dgamma < DGMAX_JBAD) || (cf == FAIL_OTHER);
+    if (jbad) {
+      pb.jac(tn, y);
+      jcur = true;
+      nje++;
+      nstlj = nst;
+    } else {
+      jcur = false;
+    }
+    pb.build_M(gamma, lu.a);
+    const bool ok = lu.factor(tm);
+    nsetups++;
+    gamrat = 1.0;
+    gammap = gamma;
+    crate = 1.0;
+    crateS = 1.0;
+    nstlp = nst;
+    return ok ? NLS_OK : NLS_RECOVER;
+  }
+
+  __device__ __forceinline__ double lsolve(double b) const {
+    double x = lu.solve(b, tm);
+    if (gamrat != 1.0) x *= 2.0 / (1.0 + gamrat);
+    return x;
+  }
+
+  // ---------------- nonlinear solve (Newton) ----------------
+  // cvNlsResidual: y = zn0 + acor, ftemp = f(tn, y), returns residual.
+  __device__ __forceinline__ double residual() {
+    y = zn[0] + acor;
+    ftemp = pb.rhs(tn, y);
+    nfe++;
+    return rl1 * zn[1] + acor - gamma * ftemp;
+  }
+
+  // SUNNonlinSolSolve_Newton with cvNlsConvTest; returns NLS_OK / NLS_RECOVER / <0.
+  __device__ __forceinline__ int newton(bool callSetup) {
+    bool jbad = false;
+    for (;;) {
+      int m = 0;
+      double res = residual();
+      int ret = NLS_OK;
+      if (!isfinite(tm.max(active ? fabs(res) : 0.0))) ret = NLS_RECOVER;
+      if (ret == NLS_OK && callSetup) {
+        if (jbad) convfail = FAIL_BAD_J;
+        ret = lsetup(convfail);
+      }
+      if (ret == NLS_OK) {
+        for (;;) {
+          nni++;
+          const double delta = lsolve(-res);
+          acor += delta;
+          const double del = wrms(delta, ewt);
+          if (m > 0) crate = fmax(CRDOWN * crate, del / delp);
+          const double dcon = del * fmin(1.0, crate) / tq[4];
+          if (dcon <= 1.0) {
+            acnrm = (m == 0) ? del : wrms(acor, ewt);
+            jcur = false;
+            return NLS_OK;
+          }
+          if (m >= 1 && del > RDIV * delp) { ret = NLS_RECOVER; break; }
+          delp = del;
+          m++;
+          if (m >= NLS_MAXCOR) { ret = NLS_RECOVER; break; }
+          res = residual();
+          if (!isfinite(tm.max(active ? fabs(res) : 0.0))) { ret = NLS_RECOVER; break; }
+        }
+      }
+      if (ret == NLS_RECOVER && !jcur) {
+        callSetup = true;
+        jbad = true;
+        acor = 0.0;
+        continue;
+      }
+      return ret;
+    }
+  }
+
+  __device__ __forceinline__ int nls(int nflag) {
+    convfail = (nflag == FIRST_CALL || nflag == PREV_ERR_FAIL) ? NO_FAILURES : FAIL_OTHER;
+    const bool callSetup = (nflag == PREV_CONV_FAIL) || (nflag == PREV_ERR_FAIL) ||
+                           (nst == 0) || (nst >= nstlp + MSBP) ||
+                           (fabs(gamrat - 1.0) > DGMAX_LSETUP);
+    acor = 0.0;
+    const int flag = newton(callSetup);
+    if (flag != NLS_OK) return flag;
+    y = zn[0] + acor;
+    jcur = false;
+    return NLS_OK;
+  }
+
+  // cvHandleNFlag
+  __device__ __forceinline__ int handle_nflag(int& nflag, double saved_t, int& ncf) {
+    if (nflag == NLS_OK) return DO_ERROR_TEST;
+    ncfn++;
+    restore(saved_t);
+    ncf++;
+    etamax = 1.0;
+    if (ncf == MXNCF) return AMX_CONV_FAILURE;
+    eta = ETA_CF;
+    nflag = PREV_CONV_FAIL;
+    rescale();
+    return PREDICT_AGAIN;
+  }
+
+  // cvDoErrorTest (state part); extra(...) re-evaluates quadrature / sensitivity
+  // derivatives on the order-1 restart branch.
+  __device__ __forceinline__ int error_test(int& nflag, double saved_t, double acor_nrm, int& nef) {
+    const double dsm = acor_nrm * tq[2];
+    dsm_test = dsm;
+    if (dsm <= 1.0) return AMX_SUCCESS;
+    nef++;
+    netf++;
+    nflag = PREV_ERR_FAIL;
+    restore(saved_t);
+    if (nef == MXNEF) return AMX_ERR_FAILURE;
+    etamax = 1.0;
+    if (nef <= MXNEF1) {
+      eta = 1.0 / (powr(BIAS2 * dsm, 1.0 / L) + ADDON);
+      eta = fmax(ETA_MIN_EF, eta);
+      if (nef >= SMALL_NEF) eta = fmin(eta, ETA_MAX_EF);
+      rescale();
+      return TRY_AGAIN;
+    }
+    if (q > 1) {
+      eta = ETA_MIN_EF;
+      adjust_order(-1);
+      L = q;
+      q--;
+      qwait = L;
+      rescale();
+      return TRY_AGAIN;
+    }
+    eta = ETA_MIN_EF;
+    h *= eta;
+    hscale = h;
+    qwait = LONG_WAIT;
+    nscon = 0;
+    const double f0 = pb.rhs(tn, zn[0]);
+    nfe++;
+    zn[1] = active ? h * f0 : 0.0;
+    if constexpr (QUAD) {
+      double fq[EPQ];
+      pb.quad_rhs(tn, zn[0], fq);
+      nfQe++;
+#pragma unroll
+      for (int e = 0; e < EPQ; ++e) znQ[1][e] = h * fq[e];
+    }
+    if constexpr (SENS) pb.sens_restart_q1(tn, zn[0], f0, h);
+    return TRY_AGAIN;
+  }
+
+  // cvQuadNls
+  __device__ __forceinline__ void quad_correct() {
+    double fq[EPQ];
+    pb.quad_rhs(tn, y, fq);
+    nfQe++;
+#pragma unroll
+    for (int e = 0; e < EPQ; ++e) {
+      acorQ[e] = rl1 * (h * fq[e] - znQ[1][e]);
+      yQ[e] = znQ[0][e] + acorQ[e];
+    }
+  }
+
+  // cvCompleteStep
+  __device__ __forceinline__ void complete_step() {
+    nst++;
+    nscon++;
+    hu = h;
+#pragma unroll
+    for (int i = QMAX; i >= 2; --i)
+      if (i <= q) tau[i] = tau[i - 1];
+    if (q == 1 && nst > 1) tau[2] = tau[1];
+    tau[1] = h;
+#pragma unroll
+    for (int j = 0; j <= QMAX; ++j)
+      if (j <= q) {
+        zn[j] = fma(l[j], acor, zn[j]);
+        if constexpr (QUAD)
+#pragma unroll
+          for (int e = 0; e < EPQ; ++e) znQ[j][e] = fma(l[j], acorQ[e], znQ[j][e]);
+      }
+    if constexpr (SENS) pb.sens_complete(q, l);
+    qwait--;
+    if (qwait == 1 && q != QMAX) {
+      zn[QMAX] = acor;
+      if constexpr (QUAD)
+#pragma unroll
+        for (int e = 0; e < EPQ; ++e) znQ[QMAX][e] = acorQ[e];
+      if constexpr (SENS) pb.sens_save_acor();
+      saved_tq5 = tq[5];
+    }
+  }
+
+  __device__ __forceinline__ void set_eta() {
+    if (eta > 0.0 && eta < ETA_MAX_FX) {
+      eta = 1.0;
+      hprime = h;
+    } else {
+      if (eta >= ETA_MAX_FX) {
+        eta = fmin(eta, etamax);
+      } else {
+        eta = fmax(eta, ETA_MIN);
+      }
+      hprime = h * eta;
+      if (qprime < q) nscon = 0;
+    }
+  }
+
+  __device__ __forceinline__ double norm_with_extras(double nstate, const double (&vQ)[EPQ], bool useQ,
+                                     double sensnorm, bool useS) const {
+    double r = nstate;
+    if constexpr (QUAD)
+      if (useQ) r = fmax(r, wrmsQ(vQ, ewtQ));
+    if (useS) r = fmax(r, sensnorm);
+    return r;
+  }
+
+  __device__ __forceinline__ double compute_etaqm1() {
+    etaqm1 = 0.0;
+    if (q > 1) {
+      double zq[EPQ];
+#pragma unroll
+      for (int e = 0; e < EPQ; ++e) {
+        zq[e] = opaque(znQ[0][e]);
+#pragma unroll
+        for (int j = 1; j < LMAX; ++j) {
+          const double v = opaque(znQ[j][e]);
+          zq[e] = (j == q) ? v : zq[e];
+        }
+      }
+      double ddn = wrms(sel(zn, q), ewt);
+      double sn = 0.0;
+      if constexpr (SENS) sn = pb.sens_norm_zn(q);
+      ddn = norm_with_extras(ddn, zq, quad_errcon(), sn, SENS) * tq[1];
+      etaqm1 = 1.0 / (powr(BIAS1 * ddn, 1.0 / q) + ADDON);
+    }
+    return etaqm1;
+  }
+
+  __device__ __forceinline__ double compute_etaqp1() {
+    etaqp1 = 0.0;
+    if (q != QMAX) {
+      if (saved_tq5 == 0.0) return etaqp1;
+      const double cquot = (tq[5] / saved_tq5) * powr(h / tau[2], (double)L);
+      const double tv = acor - cquot * zn[QMAX];
+      double tQ[EPQ];
+#pragma unroll
+      for (int e = 0; e < EPQ; ++e) tQ[e] = acorQ[e] - cquot * znQ[QMAX][e];
+      double dup = wrms(tv, ewt);
+      double sn = 0.0;
+      if constexpr (SENS) sn = pb.sens_norm_qp1(cquot);
+      dup = norm_with_extras(dup, tQ, quad_errcon(), sn, SENS) * tq[3];
+      etaqp1 = 1.0 / (powr(BIAS3 * dup, 1.0 / (L + 1)) + ADDON);
+    }
+    return etaqp1;
+  }
+
+  __device__ __forceinline__ void choose_eta() {
+    const double etam = fmax(etaqm1, fmax(etaq, etaqp1));
+    if (etam > 0.0 && etam < ETA_MAX_FX) {
+      eta = 1.0;
+      qprime = q;
+    } else if (etam == etaq) {
+      eta = etaq;
+      qprime = q;
+    } else if (etam == etaqm1) {
+      eta = etaqm1;
+      qprime = q - 1;
+    } else {
+      eta = etaqp1;
+      qprime = q + 1;
+      zn[QMAX] = acor;
+      if constexpr (QUAD)
+        if (quad_errcon())
+#pragma unroll
+          for (int e = 0; e < EPQ; ++e) znQ[QMAX][e] = acorQ[e];
+      if constexpr (SENS) pb.sens_save_acor();
+    }
+  }
+
+  __device__ __forceinline__ void prepare_next_step(double dsm) {
+    if (etamax == 1.0) {
+      qwait = max(qwait, 2);
+      qprime = q;
+      hprime = h;
+      eta = 1.0;
+      return;
+    }
+    etaq = 1.0 / (powr(BIAS2 * dsm, 1.0 / L) + ADDON);
+    if (qwait != 0) {
+      eta = etaq;
+      qprime = q;
+      set_eta();
+    } else {
+      qwait = 2;
+      compute_etaqm1();
+      compute_etaqp1();
+      choose_eta();
+      set_eta();
+    }
+  }
+
+  // ---------------- one step: cvStep ----------------
+  __device__ __forceinline__ int step() {
+    int ncf = 0, nef = 0, nefQ = 0, ncfS = 0, nefS = 0;
+    if (nst > 0 && hprime != h) adjust_params();
+    const double saved_t = tn;
+    int nflag = FIRST_CALL;
+    double dsm = 0.0;
+    for (;;) {
+      predict();
+      set_coeffs();
+      nflag = nls(nflag);
+      int kflag = handle_nflag(nflag, saved_t, ncf);
+      if (kflag == PREDICT_AGAIN) continue;
+      if (kflag != DO_ERROR_TEST) return kflag;
+      int eflag = error_test(nflag, saved_t, acnrm, nef);
+      if (eflag == TRY_AGAIN) continue;
+      if (eflag != AMX_SUCCESS) return eflag;
+      dsm = dsm_test;
+      if constexpr (QUAD) {
+        ncf = nef = 0;
+        quad_correct();
+        if (quad_errcon()) {
+          const double acnrmQ = wrmsQ(acorQ, ewtQ);
+          eflag = error_test(nflag, saved_t, acnrmQ, nefQ);
+          if (eflag == TRY_AGAIN) continue;
+          if (eflag != AMX_SUCCESS) return eflag;
+          dsm = fmax(dsm, dsm_test);
+        }
+      }
+      if constexpr (SENS) {
+        ncf = nef = 0;
+        nefQ = 0;
+        ftemp = pb.rhs(tn, y);
+        nfe++;
+        int sflag = pb.sens_nls(*this);
+        kflag = handle_nflag(sflag, saved_t, ncfS);
+        nflag = sflag;
+        if (kflag == PREDICT_AGAIN) continue;
+        if (kflag != DO_ERROR_TEST) return kflag;
+        eflag = error_test(nflag, saved_t, acnrmS, nefS);
+        if (eflag == TRY_AGAIN) continue;
+        if (eflag != AMX_SUCCESS) return eflag;
+        dsm = fmax(dsm, dsm_test);
+      }
+      break;
+    }
+    complete_step();
+    prepare_next_step(dsm);
+    etamax = (nst <= SMALL_NST) ? ETA_MAX_ES : ETA_MAX_GS;
+    dsm_last = dsm;
+    return AMX_SUCCESS;
+  }
+
+  // ---------------- initial step size: cvHin ----------------
+  __device__ __forceinline__ double upper_bound_h0(double tdist) {
+    double t1 = active ? (HUB_FACTOR * fabs(zn[0]) + 1.0 / ewt) : 1.0;
+    double hub_inv = tm.max(active ? fabs(zn[1]) / t1 : 0.0);
+    if constexpr (QUAD) {
+      if (quad_errcon()) {
+        double m = 0.0;
+#pragma unroll
+        for (int e = 0; e < EPQ; ++e) {
+          const int g = tm.lane + e * T;
+          if (g < NQ) m = fmax(m, fabs(znQ[1][e]) / (HUB_FACTOR * fabs(znQ[0][e]) + 1.0 / ewtQ[e]));
+        }
+        hub_inv = fmax(hub_inv, tm.max(m));
+      }
+    }
+    if constexpr (SENS) hub_inv = fmax(hub_inv, pb.sens_hub_inv());
+    double hub = HUB_FACTOR * tdist;
+    if (hub * hub_inv > 1.0) hub = 1.0 / hub_inv;
+    return hub;
+  }
+
+  __device__ __forceinline__ double ydd_norm(double hg) {
+    const double yy = zn[0] + hg * zn[1];
+    if constexpr (SENS) pb.sens_ydd_prepare(hg);
+    double fv = pb.rhs(tn + hg, yy);
+    nfe++;
+    double tv = (fv - zn[1]) / hg;
+    double r = wrms(tv, ewt);
+    if constexpr (QUAD) {
+      if (quad_errcon()) {
+        double fq[EPQ];
+        pb.quad_rhs(tn + hg, yy, fq);
+        nfQe++;
+        double tQ[EPQ];
+#pragma unroll
+        for (int e = 0; e < EPQ; ++e) tQ[e] = (fq[e] - znQ[1][e]) / hg;
+        r = fmax(r, wrmsQ(tQ, ewtQ));
+      }
+    }
+    if constexpr (SENS) r = fmax(r, pb.sens_ydd_norm(tn + hg, yy, fv, hg));
+    return r;
+  }
+
+  __device__ __forceinline__ int hin(double tout) {
+    const double tdiff = tout - tn;
+    if (tdiff == 0.0) return AMX_TOO_CLOSE;
+    const double sign = (tdiff > 0.0) ? 1.0 : -1.0;
+    const double tdist = fabs(tdiff);
+    const double tround = UROUND * fmax(fabs(tn), fabs(tout));
+    if (tdist < 2.0 * tround) return AMX_TOO_CLOSE;
+    const double hlb = HLB_FACTOR * tround;
+    const double hub = upper_bound_h0(tdist);
+    double hg = sqrt(hlb * hub);
+    if (hub < hlb) {
+      h = sign * hg;
+      return AMX_SUCCESS;
+    }
+    double hnew = hg, hs = hg;
+    for (int count1 = 1; count1 <= 4; ++count1) {
+      double yddnrm = 0.0;
+      bool ok = false;
+      for (int count2 = 1; count2 <= 4; ++count2) {
+        yddnrm = ydd_norm(hg * sign);
+        if (isfinite(yddnrm)) { ok = true; break; }
+        hg *= 0.2;
+      }
+      if (!ok) {
+        if (count1 <= 2) return AMX_RHSFUNC_FAIL;
+        hnew = hs;
+        break;
+      }
+      hs = hg;
+      hnew = (yddnrm * hub * hub > 2.0) ? sqrt(2.0 / yddnrm) : sqrt(hg * hub);
+      if (count1 == 4) break;
+      const double hrat = hnew / hg;
+      if (hrat > 0.5 && hrat < 2.0) break;
+      if (count1 > 1 && hrat > 2.0) { hnew = hg; break; }
+      hg = hnew;
+    }
+    double h0 = H_BIAS * hnew;
+    if (h0 < hlb) h0 = hlb;
+    if (h0 > hub) h0 = hub;
+    h = sign * h0;
+    return AMX_SUCCESS;
+  }
+
+  // ---------------- CVode(NORMAL) with tstop == tout ----------------
+  // Advances to tout; on success tn == tout and zn[0] holds y(tout).
+  // pb.on_step() is invoked for the initial point and after every accepted
+  // step (trajectory storage); returning false stops with AMX_TRAJ_FULL.
+  __device__ __forceinline__ int advance(double tout) {
+    tstop = tout;
+    tstopset = true;
+    if (nst == 0) {
+      set_ewt();
+      if constexpr (QUAD) set_ewtQ();
+      if constexpr (SENS) pb.sens_set_ewt();
+      const double f0 = pb.rhs(tn, zn[0]);
+      nfe++;
+      zn[1] = active ? f0 : 0.0;
+      if constexpr (QUAD) {
+        double fq[EPQ];
+        pb.quad_rhs(tn, zn[0], fq);
+        nfQe++;
+#pragma unroll
+        for (int e = 0; e < EPQ; ++e) znQ[1][e] = fq[e];
+      }
+      if constexpr (SENS) pb.sens_init_deriv(tn, zn[0], f0);
+      if ((tstop - tn) * (tout - tn) <= 0.0) return AMX_ILL_INPUT;
+      int hf = hin(tout);
+      if (hf != AMX_SUCCESS) return hf;
+      if ((tn + h - tstop) * h > 0.0) h = (tstop - tn) * (1.0 - 4.0 * UROUND);
+      hscale = h;
+      hprime = h;
+      zn[1] *= h;
+      if constexpr (QUAD)
+#pragma unroll
+        for (int e = 0; e < EPQ; ++e) znQ[1][e] *= h;
+      if constexpr (SENS) pb.sens_scale_deriv(h);
+      if (!pb.on_step(tn, zn[0], zn[1], hscale)) return AMX_TRAJ_FULL;
+    } else {
+      const double troundoff = FUZZ_FACTOR * UROUND * (fabs(tn) + fabs(h));
+      if (fabs(tn - tstop) <= troundoff) {
+        tstopset = false;
+        return AMX_SUCCESS;
+      }
+      if ((tn + hprime - tstop) * h > 0.0) {
+        hprime = (tstop - tn) * (1.0 - 4.0 * UROUND);
+        eta = hprime / h;
+      }
+    }
+    for (;;) {
+      if (nst > 0) {
+        set_ewt();
+        if constexpr (QUAD) if (quad_errcon()) set_ewtQ();
+        if constexpr (SENS) pb.sens_set_ewt();
+      }
+      if (nst >= sp.maxsteps) return AMX_TOO_MUCH_WORK;
+      double nrm = wrms(zn[0], ewt);
+      if constexpr (QUAD) if (quad_errcon()) nrm = fmax(nrm, wrmsQ(znQ[0], ewtQ));
+      if constexpr (SENS) nrm = fmax(nrm, pb.sens_norm_zn(0));
+      tolsf = UROUND * nrm;
+      if (tolsf > 1.0) return AMX_TOO_MUCH_ACC;
+      tolsf = 1.0;
+      const int kflag = step();
+      if (kflag != AMX_SUCCESS) return kflag;
+      const double troundoff = FUZZ_FACTOR * UROUND * (fabs(tn) + fabs(h));
+      if (fabs(tn - tstop) <= troundoff) tn = tstop;
+      if (!pb.on_step(tn, zn[0], zn[1], hscale)) return AMX_TRAJ_FULL;
+      if (tn == tstop) {
+        tstopset = false;
+        return AMX_SUCCESS;
+      }
+      if ((tn + hprime - tstop) * h > 0.0) {
+        hprime = (tstop - tn) * (1.0 - 4.0 * UROUND);
+        eta = hprime / h;
+      }
+    }
+  }
+};
+
+}  // namespace amx

@@ -1,0 +1,799 @@
+// Mass-action model evaluation, forward / forward-sensitivity / adjoint
+// problems and the persistent batch kernels.  A generated model header defines
+// a traits struct M (dimensions, team size and constexpr sparsity tables) and
+// includes this file.
+//
+// Model semantics follow AMICI's SBML mass-action import, xdot = S w with
+// w_r = p_r * prod(x_reactants) (python/sdist/amici/importers/sbml/__init__.py:620-627),
+// the Jacobian / df/dp assembly of Model_ODE (src/model_ode.cpp:14-58, 141-170),
+// sensitivity RHS (model_ode.cpp:426-450), adjoint RHS xBdot = -J^T xB and
+// quadrature qBdot = -(df/dp)^T xB (model_ode.cpp:331-374), observables y = C x,
+// Gaussian / log-normal Jy (python/sdist/amici/importers/utils.py:434-440),
+// llh / sllh assembly (src/model.cpp:1194-1239, src/rdata.cpp:514-629) and the
+// log10 chain rule (src/rdata.cpp:782-830).
+#pragma once
+#include "amx_bdf.h"
+
+namespace amx {
+
+constexpr int TPB = 256;  // threads per block for every batch kernel
+
+struct BatchArgs {
+  int nb, nt;
+  int ts_stride;           // 0: one time grid for all instances, else nt
+  const double* theta;     // [nb][NP] log10 parameters
+  const double* kfix;      // [nb][NK]
+  const double* ts;        // [nb or 1][nt]
+  const double* meas;      // [nb][nt][NY] (NaN = missing) or null
+  const double* sigma;     // [nb][nt][NY]
+  double* x;               // [nb][nt][NX] or null
+  double* y;               // [nb][nt][NY] or null
+  double* sx;              // [nb][nt][NP][NX] or null (FSA)
+  double* llh;             // [nb]
+  double* sllh;            // [nb][NP] (FSA / adjoint)
+  int* status;             // [nb]
+  int* stats;              // [nb][AMX_NSTATS] or null
+  // adjoint forward trajectory (Hermite data at every accepted step)
+  double* traj_t;          // [nb][max_traj]
+  double* traj_y;          // [nb][max_traj][NX]
+  double* traj_f;          // [nb][max_traj][NX]
+  int* traj_n;             // [nb]
+  double* dJydx;           // [nb][nt][NX]
+  int max_traj;
+  double* sens_ws;         // FSA workspace [nb][SENS_SLOTS][NP][NX]
+  int* counter;            // work-queue counter (zero before launch)
+  SolverParams sp;         // forward
+  SolverParams spB;        // backward (adjoint)
+};
+
+// stats layout: forward nst, nfe, nsetups, nje, nni, netf, ncfn, last order,
+//               backward nstB, nfeB, nsetupsB, njeB, nniB, netfB, ncfnB, nfQeB
+constexpr int AMX_NSTATS = 16;
+constexpr int SENS_SLOTS = LMAX + 3;  // znS[0..5], acorS, ewtS, sllh partials
+
+template <class M>
+struct Smem {
+  static constexpr int JLD = M::NX + 1;
+  static constexpr int NV = (M::NR > M::NY ? M::NR : M::NY) > M::NX
+                                ? (M::NR > M::NY ? M::NR : M::NY) : M::NX;
+  double x[M::NX];
+  double v[M::NX];
+  double pk[M::NR];   // rate constant of every reaction
+  double w[NV];       // fluxes / monomials / observable scratch
+  double u[NV];       // S^T v scratch
+  double J[M::NX * JLD];
+};
+
+// ---------------------------------------------------------------------------
+// model evaluation shared by all problems (team-collective)
+template <class M>
+struct MAEval {
+  static constexpr int NX = M::NX, NR = M::NR, T = M::T;
+  Team<T> tm;
+  Smem<M>& s;
+
+  __device__ __forceinline__ MAEval(const Team<T>& t, Smem<M>& sm) : tm(t), s(sm) {}
+
+  __device__ __forceinline__ void put_x(double x) {
+    team_sync();
+    if (tm.lane < NX) s.x[tm.lane] = x;
+    team_sync();
+  }
+  // monomials m_r = prod x_reactants into s.w (x already in s.x)
+  __device__ __forceinline__ void monomials() {
+    for (int r = tm.lane; r < NR; r += T) {
+      const int a = M::RA[r], b = M::RB[r];
+      double m = (a >= 0) ? s.x[a] : 1.0;
+      if (b >= 0) m *= s.x[b];
+      s.w[r] = m;
+    }
+    team_sync();
+  }
+  __device__ __forceinline__ double xdot(double x) {
+    put_x(x);
+    for (int r = tm.lane; r < NR; r += T) {
+      const int a = M::RA[r], b = M::RB[r];
+      double m = s.pk[r];
+      if (a >= 0) m *= s.x[a];
+      if (b >= 0) m *= s.x[b];
+      s.w[r] = m;
+    }
+    team_sync();
+    double f = 0.0;
+    if (tm.lane < NX)
+      for (int q = M::SP_PTR[tm.lane]; q < M::SP_PTR[tm.lane + 1]; ++q)
+        f = fma(M::SP_C[q], s.w[M::SP_R[q]], f);
+    return f;
+  }
+  // dense J into s.J (row-major, leading dim JLD); x already in s.x
+  __device__ __forceinline__ void jacobian() {
+    constexpr int JLD = Smem<M>::JLD;
+    for (int e = tm.lane; e < NX * NX; e += T) s.J[(e / NX) * JLD + (e % NX)] = 0.0;
+    team_sync();
+    for (int e = tm.lane; e < M::JNZ; e += T) {
+      double v = 0.0;
+      for (int t = M::JT_PTR[e]; t < M::JT_PTR[e + 1]; ++t) {
+        double d = M::JT_C[t] * s.pk[M::JT_R[t]];
+        const int o = M::JT_O[t];
+        if (o >= 0) d *= s.x[o];
+        v += d;
+      }
+      s.J[M::J_I[e] * JLD + M::J_J[e]] = v;
+    }
+    team_sync();
+  }
+  // u_r = sum_i S_ir v_i into s.u; v in s.v
+  __device__ __forceinline__ void stoich_t() {
+    for (int r = tm.lane; r < NR; r += T) {
+      double u = 0.0;
+      for (int q = M::RS_PTR[r]; q < M::RS_PTR[r + 1]; ++q)
+        u = fma(M::RS_C[q], s.v[M::RS_S[q]], u);
+      s.u[r] = u;
+    }
+    team_sync();
+  }
+};
+
+// ---------------------------------------------------------------------------
+// forward problem, optionally with NS staggered forward sensitivities
+template <class M, int NS_>
+struct FwdProblem {
+  static constexpr int NX = M::NX, T = M::T, NQ = 0, NS = NS_, NP = M::NP;
+  static constexpr int JLD = Smem<M>::JLD;
+  Team<T> tm;
+  Smem<M>& s;
+  MAEval<M> ev;
+  double* ws;  // FSA workspace of this instance: [SENS_SLOTS][NS][NX]
+  // adjoint forward pass: Hermite data of every accepted step
+  double* tt;
+  double* ty;
+  double* tf;
+  int ntraj, max_traj;
+
+  __device__ __forceinline__ FwdProblem(const Team<T>& t, Smem<M>& sm, double* w)
+      : tm(t), s(sm), ev(t, sm), ws(w), tt(nullptr), ty(nullptr), tf(nullptr), ntraj(0),
+        max_traj(0) {}
+
+  __device__ __forceinline__ bool on_step(double tn, double z0, double z1, double hs) {
+    if (tt == nullptr) return true;
+    if (ntraj >= max_traj) return false;
+    if (tm.lane == 0) tt[ntraj] = tn;
+    if (tm.lane < NX) {
+      ty[(size_t)ntraj * NX + tm.lane] = z0;
+      tf[(size_t)ntraj * NX + tm.lane] = z1 / hs;
+    }
+    ntraj++;
+    return true;
+  }
+
+  __device__ __forceinline__ double rhs(double, double y) { return ev.xdot(y); }
+  __device__ __forceinline__ void quad_rhs(double, double, double*) {}
+  __device__ __forceinline__ void jac(double, double y) {
+    ev.put_x(y);
+    ev.jacobian();
+  }
+  __device__ __forceinline__ void build_M(double gamma, double* a) {
+    const int i = tm.lane;
+#pragma unroll
+    for (int j = 0; j < NX; ++j)
+      a[j] = (i < NX) ? ((i == j ? 1.0 : 0.0) - gamma * s.J[i * JLD + j]) : 0.0;
+  }
+
+  // ---------- forward sensitivities (global-memory Nordsieck arrays) ----------
+  __device__ __forceinline__ double& S(int slot, int is) const {
+    const int i = tm.lane < NX ? tm.lane : NX - 1;
+    return ws[((size_t)slot * NS + is) * NX + i];
+  }
+  static constexpr int ACOR = LMAX, EWT = LMAX + 1, ACC = LMAX + 2;
+  double ssp_rtol, ssp_atol;
+
+  // Stages y (for J*v through the flux structure) and the monomials (df/dp).
+  __device__ __forceinline__ void sens_prepare(double y) {
+    ev.put_x(y);
+    ev.monomials();
+  }
+  // sensitivity RHS for column is at the staged y:
+  //   (J v)_i + df_i/dp_is = sum_r S_ir (dw_r/dx . v + [KI(r) == is] m_r)
+  __device__ __forceinline__ double sens_f(double yS, int is) {
+    team_sync();
+    if (tm.lane < NX) s.v[tm.lane] = yS;
+    team_sync();
+    for (int r = tm.lane; r < M::NR; r += T) {
+      const int a = M::RA[r], b = M::RB[r];
+      double d = 0.0;
+      if (a >= 0) d = s.v[a] * ((b >= 0) ? s.x[b] : 1.0);
+      if (b >= 0) d = fma(s.v[b], s.x[a], d);
+      double u = s.pk[r] * d;
+      if (M::KI[r] == is) u += s.w[r];
+      s.u[r] = u;
+    }
+    team_sync();
+    double acc = 0.0;
+    if (tm.lane < NX)
+      for (int q = M::SP_PTR[tm.lane]; q < M::SP_PTR[tm.lane + 1]; ++q)
+        acc = fma(M::SP_C[q], s.u[M::SP_R[q]], acc);
+    return acc;
+  }
+  __device__ __forceinline__ double sens_norm_slot(int slot) const {
+    double m = 0.0;
+    for (int is = 0; is < NS; ++is) {
+      const double e = (tm.lane < NX) ? S(slot, is) * S(EWT, is) : 0.0;
+      m = fmax(m, sqrt(tm.sum(e * e) / NX));
+    }
+    return m;
+  }
+  __device__ __forceinline__ void sens_init() {
+    if (tm.lane < NX)
+      for (int sl = 0; sl < SENS_SLOTS; ++sl)
+        for (int is = 0; is < NS; ++is) S(sl, is) = 0.0;  // sx0 = dx0/dp = 0
+  }
+  __device__ __forceinline__ void sens_set_ewt() {
+    if (tm.lane < NX)
+      for (int is = 0; is < NS; ++is) S(EWT, is) = 1.0 / (ssp_rtol * fabs(S(0, is)) + ssp_atol);
+  }
+  __device__ __forceinline__ void sens_predict(int q) {
+    if (tm.lane >= NX) return;
+    for (int is = 0; is < NS; ++is) {
+      double z[LMAX];
+#pragma unroll
+      for (int j = 0; j < LMAX; ++j) z[j] = (j <= q) ? S(j, is) : 0.0;
+#pragma unroll
+      for (int k = 1; k <= QMAX; ++k)
+#pragma unroll
+        for (int j = QMAX; j >= k; --j)
+          if (k <= q && j <= q) z[j - 1] += z[j];
+#pragma unroll
+      for (int j = 0; j < QMAX; ++j)
+        if (j < q) S(j, is) = z[j];
+    }
+  }
+  __device__ __forceinline__ void sens_restore(int q) {
+    if (tm.lane >= NX) return;
+    for (int is = 0; is < NS; ++is) {
+      double z[LMAX];
+#pragma unroll
+      for (int j = 0; j < LMAX; ++j) z[j] = (j <= q) ? S(j, is) : 0.0;
+#pragma unroll
+      for (int k = 1; k <= QMAX; ++k)
+#pragma unroll
+        for (int j = QMAX; j >= k; --j)
+          if (k <= q && j <= q) z[j - 1] -= z[j];
+#pragma unroll
+      for (int j = 0; j < QMAX; ++j)
+        if (j < q) S(j, is) = z[j];
+    }
+  }
+  __device__ __forceinline__ void sens_rescale(int q, double eta) {
+    if (tm.lane >= NX) return;
+    for (int is = 0; is < NS; ++is) {
+      double c = eta;
+#pragma unroll
+      for (int j = 1; j <= QMAX; ++j) {
+        if (j <= q) S(j, is) *= c;
+        c *= eta;
+      }
+    }
+  }
+  __device__ __forceinline__ void sens_increase(int q, double A1, const double (&ll)[LMAX]) {
+    if (tm.lane >= NX) return;
+    for (int is = 0; is < NS; ++is) {
+      const double zL = A1 * S(QMAX, is);
+#pragma unroll
+      for (int j = 2; j <= QMAX; ++j)
+        if (j <= q) S(j, is) = fma(ll[j], zL, S(j, is));
+      S(q + 1, is) = zL;
+    }
+  }
+  __device__ __forceinline__ void sens_decrease(int q, const double (&ll)[LMAX]) {
+    if (tm.lane >= NX) return;
+    for (int is = 0; is < NS; ++is) {
+      const double zq = S(q, is);
+#pragma unroll
+      for (int j = 2; j < QMAX; ++j)
+        if (j < q) S(j, is) = fma(-ll[j], zq, S(j, is));
+    }
+  }
+  __device__ __forceinline__ void sens_complete(int q, const double (&l)[LMAX]) {
+    if (tm.lane >= NX) return;
+    for (int is = 0; is < NS; ++is) {
+      const double a = S(ACOR, is);
+#pragma unroll
+      for (int j = 0; j <= QMAX; ++j)
+        if (j <= q) S(j, is) = fma(l[j], a, S(j, is));
+    }
+  }
+  __device__ __forceinline__ void sens_save_acor() {
+    if (tm.lane >= NX) return;
+    for (int is = 0; is < NS; ++is) S(QMAX, is) = S(ACOR, is);
+  }
+  __device__ __forceinline__ double sens_norm_zn(int j) { return sens_norm_slot(j); }
+  __device__ __forceinline__ double sens_norm_qp1(double cquot) {
+    double m = 0.0;
+    for (int is = 0; is < NS; ++is) {
+      const double e = (tm.lane < NX) ? (S(ACOR, is) - cquot * S(QMAX, is)) * S(EWT, is) : 0.0;
+      m = fmax(m, sqrt(tm.sum(e * e) / NX));
+    }
+    return m;
+  }
+  __device__ __forceinline__ double sens_hub_inv() {
+    double m = 0.0;
+    if (tm.lane < NX)
+      for (int is = 0; is < NS; ++is)
+        m = fmax(m, fabs(S(1, is)) / (0.1 * fabs(S(0, is)) + 1.0 / S(EWT, is)));
+    return tm.max(m);
+  }
+  // znS[1] = fS(t, y, znS[0]) (unscaled)
+  __device__ __forceinline__ void sens_init_deriv(double, double y, double) {
+    sens_prepare(y);
+    for (int is = 0; is < NS; ++is) {
+      const double v = sens_f(tm.lane < NX ? S(0, is) : 0.0, is);
+      if (tm.lane < NX) S(1, is) = v;
+    }
+  }
+  __device__ __forceinline__ void sens_scale_deriv(double h) {
+    if (tm.lane < NX)
+      for (int is = 0; is < NS; ++is) S(1, is) *= h;
+  }
+  __device__ __forceinline__ void sens_restart_q1(double t, double y, double f, double h) {
+    sens_init_deriv(t, y, f);
+    sens_scale_deriv(h);
+  }
+  double ydd_hg;
+  __device__ __forceinline__ void sens_ydd_prepare(double hg) { ydd_hg = hg; }
+  __device__ __forceinline__ double sens_ydd_norm(double, double yy, double, double hg) {
+    sens_prepare(yy);
+    double m = 0.0;
+    for (int is = 0; is < NS; ++is) {
+      const double ys = (tm.lane < NX) ? S(0, is) + hg * S(1, is) : 0.0;
+      const double fs = sens_f(ys, is);
+      const double e = (tm.lane < NX) ? (fs - S(1, is)) / hg * S(EWT, is) : 0.0;
+      m = fmax(m, sqrt(tm.sum(e * e) / NX));
+    }
+    return m;
+  }
+
+  // cvStgrNls: Newton on all sensitivity corrections with the state's
+  // Newton matrix (no setup unless the iteration fails with a stale J).
+  template <class B>
+  __device__ __forceinline__ int sens_nls(B& b) {
+    sens_prepare(b.y);
+    bool callSetup = false, jbad = false;
+    for (;;) {
+      if (tm.lane < NX)
+        for (int is = 0; is < NS; ++is) S(ACOR, is) = 0.0;
+      if (callSetup) {
+        if (jbad) b.convfail = FAIL_BAD_J;
+        const int r = b.lsetup(b.convfail);
+        sens_prepare(b.y);
+        if (r != NLS_OK) return NLS_RECOVER;
+      }
+      int m = 0;
+      int ret = NLS_OK;
+      for (;;) {
+        b.nniS++;
+        double Del = 0.0;
+        for (int is = 0; is < NS; ++is) {
+          const double acS = (tm.lane < NX) ? S(ACOR, is) : 0.0;
+          const double yS = (tm.lane < NX) ? S(0, is) + acS : 0.0;
+          const double fS = sens_f(yS, is);
+          const double res = (tm.lane < NX) ? b.rl1 * S(1, is) + acS - b.gamma * fS : 0.0;
+          const double d = b.lsolve(-res);
+          if (tm.lane < NX) S(ACOR, is) = acS + d;
+          const double e = (tm.lane < NX) ? d * S(EWT, is) : 0.0;
+          Del = fmax(Del, sqrt(tm.sum(e * e) / NX));
+        }
+        if (m > 0) b.crateS = fmax(CRDOWN * b.crateS, Del / b.delp);
+        const double dcon = Del * fmin(1.0, b.crateS) / b.tq[4];
+        if (dcon <= 1.0) {
+          b.acnrmS = (m == 0) ? Del : sens_norm_slot(ACOR);
+          b.jcur = false;
+          return NLS_OK;
+        }
+        if (m >= 1 && Del > RDIV * b.delp) { ret = NLS_RECOVER; break; }
+        b.delp = Del;
+        m++;
+        if (m >= NLS_MAXCOR) { ret = NLS_RECOVER; break; }
+      }
+      if (ret == NLS_RECOVER && !b.jcur) {
+        callSetup = true;
+        jbad = true;
+        continue;
+      }
+      return ret;
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// adjoint problem: xB' = -J(x(t))^T xB, qB' = -(df/dp)^T xB, x(t) by cubic
+// Hermite interpolation of the stored forward steps.
+template <class M>
+struct BwdProblem {
+  static constexpr int NX = M::NX, T = M::T, NQ = M::NP, NS = 0, NP = M::NP;
+  static constexpr int JLD = Smem<M>::JLD;
+  static constexpr int EPQ = (NQ + T - 1) / T;
+  Team<T> tm;
+  Smem<M>& s;
+  MAEval<M> ev;
+  const double* tt;  // forward step times (increasing)
+  const double* ty;  // [n][NX]
+  const double* tf;  // [n][NX]
+  int n;             // stored points
+  int cur;           // interval [cur-1, cur]
+  double ta, tb, ya, yb, fa, fb;
+  double t_cached, x_cached;
+
+  __device__ __forceinline__ bool on_step(double, double, double, double) { return true; }
+
+  __device__ __forceinline__ BwdProblem(const Team<T>& t, Smem<M>& sm, const double* tt_,
+                                        const double* ty_, const double* tf_, int n_)
+      : tm(t), s(sm), ev(t, sm), tt(tt_), ty(ty_), tf(tf_), n(n_) {
+    cur = -1;
+    t_cached = -1.0;
+    x_cached = 0.0;
+  }
+
+  __device__ __forceinline__ void load(int k) {
+    cur = k;
+    ta = tt[k - 1];
+    tb = tt[k];
+    const int i = tm.lane < NX ? tm.lane : 0;
+    ya = ty[(size_t)(k - 1) * NX + i];
+    yb = ty[(size_t)k * NX + i];
+    fa = tf[(size_t)(k - 1) * NX + i];
+    fb = tf[(size_t)k * NX + i];
+  }
+
+  __device__ __forceinline__ double xf(double t) {
+    if (t == t_cached && cur > 0) return x_cached;
+    if (n < 2) {
+      x_cached = ty[tm.lane < NX ? tm.lane : 0];
+      t_cached = t;
+      return x_cached;
+    }
+    int k = (cur > 0) ? cur : n - 1;
+    while (k > 1 && t < tt[k - 1]) --k;
+    while (k < n - 1 && t > tt[k]) ++k;
+    if (k != cur) load(k);
+    const double hk = tb - ta;
+    const double sx = (t - ta) / hk;
+    const double om = 1.0 - sx;
+    const double h00 = (1.0 + 2.0 * sx) * om * om;
+    const double h10 = sx * om * om;
+    const double h01 = sx * sx * (3.0 - 2.0 * sx);
+    const double h11 = sx * sx * (sx - 1.0);
+    x_cached = h00 * ya + h10 * hk * fa + h01 * yb + h11 * hk * fb;
+    t_cached = t;
+    return x_cached;
+  }
+
+  // puts x(t) and the adjoint state into LDS and computes u = S^T xB
+  __device__ __forceinline__ void stage(double t, double yB) {
+    const double x = xf(t);
+    team_sync();
+    if (tm.lane < NX) {
+      s.x[tm.lane] = x;
+      s.v[tm.lane] = yB;
+    }
+    team_sync();
+    ev.stoich_t();
+  }
+
+  __device__ __forceinline__ double rhs(double t, double yB) {
+    stage(t, yB);
+    double f = 0.0;
+    if (tm.lane < NX)
+      for (int q = M::DX_PTR[tm.lane]; q < M::DX_PTR[tm.lane + 1]; ++q) {
+        const int r = M::DX_R[q], o = M::DX_O[q];
+        double d = M::DX_C[q] * s.pk[r];
+        if (o >= 0) d *= s.x[o];
+        f = fma(-d, s.u[r], f);
+      }
+    return f;
+  }
+
+  __device__ __forceinline__ void quad_rhs(double t, double yB, double* fq) {
+    stage(t, yB);
+    ev.monomials();
+#pragma unroll
+    for (int e = 0; e < EPQ; ++e) {
+      const int j = tm.lane + e * T;
+      double acc = 0.0;
+      if (j < NP)
+        for (int q = M::PR_PTR[j]; q < M::PR_PTR[j + 1]; ++q) {
+          const int r = M::PR_R[q];
+          acc = fma(-s.w[r], s.u[r], acc);
+        }
+      fq[e] = acc;
+    }
+  }
+
+  __device__ __forceinline__ void jac(double t, double) {
+    const double x = xf(t);
+    ev.put_x(x);
+    ev.jacobian();
+  }
+  // Newton matrix of the backward problem: I - gamma * (-J^T)
+  __device__ __forceinline__ void build_M(double gamma, double* a) {
+    const int i = tm.lane;
+#pragma unroll
+    for (int j = 0; j < NX; ++j)
+      a[j] = (i < NX) ? ((i == j ? 1.0 : 0.0) + gamma * s.J[j * JLD + i]) : 0.0;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// observables and objective at one output time; returns this team's Jy sum
+// contribution (already reduced) and leaves dJy/dx of this lane in dJdx.
+struct ObjOut {
+  double nllh;  // team sum of Jy at this output time
+  double dJdx;  // dJy/dx of this lane's state
+};
+template <class M>
+__device__ __forceinline__ ObjOut objective_at(const Team<M::T>& tm, Smem<M>& s, double x,
+                                               const double* meas, const double* sigma,
+                                               double* yout) {
+  constexpr int NY = M::NY, NX = M::NX, T = M::T;
+  team_sync();
+  if (tm.lane < NX) s.x[tm.lane] = x;
+  team_sync();
+  double nllh = 0.0;
+  for (int k = tm.lane; k < NY; k += T) {
+    double yk = 0.0;
+    for (int q = M::OB_PTR[k]; q < M::OB_PTR[k + 1]; ++q) yk = fma(M::OB_C[q], s.x[M::OB_S[q]], yk);
+    if (yout) yout[k] = yk;
+    double g = 0.0;
+    const double mv = meas ? meas[k] : NAN;
+    if (!isnan(mv)) {
+      const double sg = sigma[k];
+      if (M::NOISE == 1) {
+        const double r = (log(yk) - log(mv)) / sg;
+        nllh += 0.5 * log(2.0 * M_PI * sg * sg * mv * mv) + 0.5 * r * r;
+        g = r / (sg * yk);
+      } else {
+        const double r = (yk - mv) / sg;
+        nllh += 0.5 * log(2.0 * M_PI * sg * sg) + 0.5 * r * r;
+        g = r / sg;
+      }
+    }
+    s.w[k] = g;
+  }
+  team_sync();
+  double d = 0.0;
+  if (tm.lane < NX)
+    for (int q = M::OT_PTR[tm.lane]; q < M::OT_PTR[tm.lane + 1]; ++q)
+      d = fma(M::OT_C[q], s.w[M::OT_Y[q]], d);
+  team_sync();
+  return ObjOut{tm.sum(nllh), d};
+}
+
+__device__ __forceinline__ bool next_t_too_close(double cur, double next) {
+  const double tdiff = next - cur;
+  if (tdiff == 0.0) return true;
+  const double tround = UROUND * fmax(fabs(cur), fabs(next));
+  return fabs(tdiff) < 2.0 * tround;
+}
+
+template <class B>
+__device__ __forceinline__ void store_stats(int* st, const B& b, int off) {
+  st[off + 0] = b.nst;
+  st[off + 1] = b.nfe;
+  st[off + 2] = b.nsetups;
+  st[off + 3] = b.nje;
+  st[off + 4] = b.nni;
+  st[off + 5] = b.netf;
+  st[off + 6] = b.ncfn;
+  st[off + 7] = (off == 0) ? b.q : b.nfQe;
+}
+
+// ---------------------------------------------------------------------------
+// forward simulation of one instance.  MODE 0: states, 1: FSA, 2: adjoint
+// forward pass (stores the trajectory and dJy/dx at every output).
+template <class M, int MODE>
+__device__ __forceinline__ void simulate_forward(int inst, const BatchArgs& a, const Team<M::T>& tm, Smem<M>& s) {
+  constexpr int NX = M::NX, NP = M::NP, NK = M::NK, NY = M::NY, T = M::T;
+  constexpr int NS = (MODE == 1) ? NP : 0;
+  const size_t I = (size_t)inst;
+  for (int r = tm.lane; r < M::NR; r += T) s.pk[r] = exp10(a.theta[I * NP + M::KI[r]]);
+  team_sync();
+  double x0 = 0.0;
+  if (tm.lane < NX) {
+    const int kx = M::X0K[tm.lane];
+    x0 = (kx >= 0) ? a.kfix[I * NK + kx] : M::X0C[tm.lane];
+  }
+  double* ws = (MODE == 1) ? a.sens_ws + I * (size_t)SENS_SLOTS * NP * NX : nullptr;
+  using PB = FwdProblem<M, NS>;
+  PB pb0(tm, s, ws);
+  pb0.ssp_rtol = a.sp.rtolS;
+  pb0.ssp_atol = a.sp.atolS;
+  if constexpr (MODE == 2) {
+    pb0.tt = a.traj_t + I * (size_t)a.max_traj;
+    pb0.ty = a.traj_y + I * (size_t)a.max_traj * NX;
+    pb0.tf = a.traj_f + I * (size_t)a.max_traj * NX;
+    pb0.max_traj = a.max_traj;
+  }
+  BDF<PB> bdf(pb0, a.sp);
+  PB& pb = bdf.pb;
+  bdf.init(0.0, x0, nullptr);
+  if constexpr (MODE == 1) pb.sens_init();
+  const double* ts = a.ts + (size_t)a.ts_stride * I;
+
+  int status = AMX_SUCCESS;
+  double llh = 0.0;
+  int it = 0;
+  for (; it < a.nt; ++it) {
+    const double tout = ts[it];
+    if (tout > 0.0 && bdf.tn < tout && !next_t_too_close(bdf.tn, tout)) {
+      status = bdf.advance(tout);
+      if (status != AMX_SUCCESS) break;
+    }
+    const double x = bdf.zn[0];
+    if (a.x && tm.lane < NX) a.x[(I * a.nt + it) * NX + tm.lane] = x;
+    const size_t mo = (I * a.nt + it) * NY;
+    const ObjOut ob = objective_at<M>(tm, s, x, a.meas ? a.meas + mo : nullptr, a.sigma + mo,
+                                      a.y ? a.y + mo : nullptr);
+    llh -= ob.nllh;
+    const double dJdx = ob.dJdx;
+    if constexpr (MODE == 2) {
+      if (tm.lane < NX) a.dJydx[(I * a.nt + it) * NX + tm.lane] = dJdx;
+    }
+    if constexpr (MODE == 1) {
+      if (tm.lane < NX)
+        for (int is = 0; is < NP; ++is) {
+          const double sxv = pb.S(0, is);
+          pb.S(PB::ACC, is) -= dJdx * sxv;
+          if (a.sx) a.sx[((I * a.nt + it) * NP + is) * NX + tm.lane] = sxv;
+        }
+    }
+  }
+  if (status != AMX_SUCCESS) {
+    llh = NAN;
+    for (int j = it; j < a.nt; ++j) {
+      if (a.x && tm.lane < NX) a.x[(I * a.nt + j) * NX + tm.lane] = NAN;
+      if (a.y)
+        for (int k = tm.lane; k < NY; k += T) a.y[(I * a.nt + j) * NY + k] = NAN;
+    }
+  }
+  if (tm.lane == 0) {
+    a.llh[inst] = llh;
+    a.status[inst] = status;
+    if (a.stats) store_stats(a.stats + I * AMX_NSTATS, bdf, 0);
+    if constexpr (MODE == 2) a.traj_n[inst] = pb.ntraj;
+  }
+  if constexpr (MODE == 1) {
+    // sllh (log10 scale) and sx chain rule (rdata.cpp:782-866)
+    for (int is = 0; is < NP; ++is) {
+      const double part = tm.sum(tm.lane < NX ? pb.S(PB::ACC, is) : 0.0);
+      const double pv = exp10(a.theta[I * NP + is]);
+      const double v = (status == AMX_SUCCESS) ? (part != 0.0 ? part * pv * M_LN10 : 0.0) : NAN;
+      if (tm.lane == 0) a.sllh[I * NP + is] = v;
+      if (a.sx && tm.lane < NX)
+        for (int j = 0; j < a.nt; ++j) {
+          double& e = a.sx[((I * a.nt + j) * NP + is) * NX + tm.lane];
+          e = (j < it) ? (e != 0.0 ? e * pv * M_LN10 : 0.0) : NAN;
+        }
+    }
+  }
+}
+
+// backward (adjoint) sweep of one instance; EventHandlingBwdSimulator::run
+// (src/backwardproblem.cpp:267-332) with a fresh order-1 restart after every
+// data-point jump (CVodeSolver::reinit_b, src/solver_cvodes.cpp:594-649).
+template <class M>
+__device__ __forceinline__ void simulate_backward(int inst, const BatchArgs& a, const Team<M::T>& tm, Smem<M>& s) {
+  constexpr int NX = M::NX, NP = M::NP, T = M::T;
+  using PB = BwdProblem<M>;
+  constexpr int EPQ = PB::EPQ;
+  const size_t I = (size_t)inst;
+  const int fst = a.status[inst];
+  if (fst != AMX_SUCCESS) {
+    for (int j = tm.lane; j < NP; j += T) a.sllh[I * NP + j] = NAN;
+    return;
+  }
+  for (int r = tm.lane; r < M::NR; r += T) s.pk[r] = exp10(a.theta[I * NP + M::KI[r]]);
+  team_sync();
+  const int n = a.traj_n[inst];
+  const double* tt = a.traj_t + I * (size_t)a.max_traj;
+  PB pb0(tm, s, tt, a.traj_y + I * (size_t)a.max_traj * NX,
+         a.traj_f + I * (size_t)a.max_traj * NX, n);
+  BDF<PB> bdf(pb0, a.spB);
+  const double* ts = a.ts + (size_t)a.ts_stride * I;
+  const double* dJ = a.dJydx + I * a.nt * NX;
+  const double t0 = 0.0;
+  int it = a.nt - 1;
+  double tcur = (n > 0) ? tt[n - 1] : t0;
+  double xB = 0.0;
+  double qB[EPQ];
+#pragma unroll
+  for (int e = 0; e < EPQ; ++e) qB[e] = 0.0;
+  if (it >= 0 && ts[it] == tcur) {
+    if (tm.lane < NX) xB += dJ[(size_t)it * NX + tm.lane];
+    --it;
+  }
+  int status = AMX_SUCCESS;
+  int st0 = 0, st1 = 0, st2 = 0, st3 = 0, st4 = 0, st5 = 0, st6 = 0, st7 = 0;
+#define AMX_ACC_STATS()                                                        \
+  do {                                                                         \
+    st0 += bdf.nst; st1 += bdf.nfe; st2 += bdf.nsetups; st3 += bdf.nje;        \
+    st4 += bdf.nni; st5 += bdf.netf; st6 += bdf.ncfn; st7 += bdf.nfQe;         \
+  } while (0)
+  bdf.init(tcur, xB, qB);
+  while (it >= 0 && status == AMX_SUCCESS) {
+    const double tnext = ts[it];
+    if (tnext < tcur) {
+      status = bdf.advance(tnext);
+      if (status != AMX_SUCCESS) break;
+      tcur = tnext;
+      xB = bdf.zn[0];
+#pragma unroll
+      for (int e = 0; e < EPQ; ++e) qB[e] = bdf.znQ[0][e];
+    }
+    if (tm.lane < NX) xB += dJ[(size_t)it * NX + tm.lane];
+    --it;
+    AMX_ACC_STATS();
+    bdf.init(tcur, xB, qB);
+  }
+  if (status == AMX_SUCCESS && tcur > t0) {
+    status = bdf.advance(t0);
+    if (status == AMX_SUCCESS) {
+#pragma unroll
+      for (int e = 0; e < EPQ; ++e) qB[e] = bdf.znQ[0][e];
+    }
+  }
+  AMX_ACC_STATS();
+#undef AMX_ACC_STATS
+  // sllh = -(xB(t0).sx0 + qB) with sx0 = 0; log10 chain rule on non-zero entries
+#pragma unroll
+  for (int e = 0; e < EPQ; ++e) {
+    const int j = tm.lane + e * T;
+    if (j < NP) {
+      const double g = -qB[e];
+      const double pv = exp10(a.theta[I * NP + j]);
+      a.sllh[I * NP + j] = (status == AMX_SUCCESS) ? (g != 0.0 ? g * pv * M_LN10 : 0.0) : NAN;
+    }
+  }
+  if (tm.lane == 0) {
+    if (status != AMX_SUCCESS) a.status[inst] = status;
+    if (a.stats) {
+      int* o = a.stats + I * AMX_NSTATS + 8;
+      o[0] = st0; o[1] = st1; o[2] = st2; o[3] = st3; o[4] = st4; o[5] = st5; o[6] = st6; o[7] = st7;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// persistent kernels: every team pulls instance indices from a device counter
+template <class M>
+__device__ __forceinline__ int next_instance(const Team<M::T>& tm, int* counter) {
+  int v = 0;
+  if (tm.lane == 0) v = atomicAdd(counter, 1);
+  return tm.bcast_i(v, 0);
+}
+
+template <class M, int MODE>
+__device__ __forceinline__ void forward_kernel_body(BatchArgs a) {
+  constexpr int T = M::T;
+  __shared__ Smem<M> smem[TPB / T];
+  Team<T> tm;
+  Smem<M>& s = smem[threadIdx.x / T];
+  for (;;) {
+    const int inst = next_instance<M>(tm, a.counter);
+    if (inst >= a.nb) break;
+    simulate_forward<M, MODE>(inst, a, tm, s);
+  }
+}
+
+template <class M>
+__device__ __forceinline__ void backward_kernel_body(BatchArgs a) {
+  constexpr int T = M::T;
+  __shared__ Smem<M> smem[TPB / T];
+  Team<T> tm;
+  Smem<M>& s = smem[threadIdx.x / T];
+  for (;;) {
+    const int inst = next_instance<M>(tm, a.counter);
+    if (inst >= a.nb) break;
+    simulate_backward<M>(inst, a, tm, s);
+  }
+}
+
+}  // namespace amx

@@ -1,0 +1,401 @@
+"""Drop-in simulation API: Model / Solver / ExpData / ReturnData and
+``run_simulations``.
+
+Mirrors the reference's Python entry points for ODE models:
+``amici.sim.sundials.run_simulations(model, solver, edata_list, failfast=True,
+num_threads=1)`` (python/sdist/amici/sim/sundials/_swig_wrappers.py:87-133,
+formerly ``runAmiciSimulations``), ``run_simulation`` (:46-84), the
+``ReturnDataView`` field layout (python/sdist/amici/sim/sundials/_numpy.py:414-451)
+and the Solver / Model / ExpData setters used on this path.  The whole list of
+conditions is run as one batch on the GPU through the C ABI
+(``include/amx.h``); ``num_threads`` is accepted and ignored.
+"""
+from __future__ import annotations
+
+import enum
+import math
+from dataclasses import dataclass, field
+from typing import Sequence
+
+import numpy as np
+import torch
+
+from . import _lib
+from .networks import ReactionNetwork
+
+AMICI_SUCCESS = 0
+AMICI_NOT_RUN = -1001
+
+
+class SensitivityOrder(enum.IntEnum):
+    none = 0
+    first = 1
+
+
+class SensitivityMethod(enum.IntEnum):
+    none = 0
+    forward = 1
+    adjoint = 2
+
+
+def _device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("amici_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+class Model:
+    """A compiled mass-action model (network + HIP model module)."""
+
+    def __init__(self, network: ReactionNetwork, library: str | None = None,
+                 team: int | None = None):
+        from . import codegen
+
+        self.network = network
+        path = library or codegen.compile_model(network, team)
+        self.lib = _lib.ModelLib(path)
+        if (self.lib.nx, self.lib.np, self.lib.nk, self.lib.ny) != (
+                network.nx, network.np, network.nk, network.ny):
+            raise RuntimeError(f"{path} was built for a different network")
+        self._theta = np.asarray(network.theta_nominal, np.float64).copy()
+        self._k = np.asarray(network.k_nominal, np.float64).copy()
+        self._ts = np.zeros(0)
+        self._ws: dict = {}
+
+    # dimensions
+    @property
+    def nx_rdata(self) -> int:
+        return self.network.nx
+
+    @property
+    def ny(self) -> int:
+        return self.network.ny
+
+    @property
+    def np(self) -> int:
+        return self.network.np
+
+    @property
+    def nk(self) -> int:
+        return self.network.nk
+
+    ne = 0
+
+    def nplist(self) -> int:
+        return self.network.np
+
+    # ids
+    def get_state_ids(self):
+        return list(self.network.species)
+
+    def get_observable_ids(self):
+        return [f"y{k}" for k in range(self.ny)]
+
+    def get_free_parameter_ids(self):
+        return list(self.network.parameter_names)
+
+    def get_fixed_parameter_ids(self):
+        return list(self.network.fixed_parameter_names)
+
+    def get_parameter_list(self):
+        return list(range(self.np))
+
+    def get_parameter_scale(self):
+        return ["log10"] * self.np
+
+    # values (free parameters in their log10 scale, as AMICI stores them)
+    def set_free_parameters(self, theta: Sequence[float]) -> None:
+        theta = np.asarray(theta, np.float64)
+        if theta.shape != (self.np,):
+            raise ValueError(f"expected {self.np} free parameters")
+        self._theta = theta.copy()
+
+    def get_free_parameters(self) -> np.ndarray:
+        return self._theta.copy()
+
+    def set_fixed_parameters(self, k: Sequence[float]) -> None:
+        k = np.asarray(k, np.float64)
+        if k.shape != (self.nk,):
+            raise ValueError(f"expected {self.nk} fixed parameters")
+        self._k = k.copy()
+
+    def get_fixed_parameters(self) -> np.ndarray:
+        return self._k.copy()
+
+    def set_timepoints(self, ts: Sequence[float]) -> None:
+        self._ts = np.asarray(ts, np.float64).copy()
+
+    def get_timepoints(self) -> np.ndarray:
+        return self._ts.copy()
+
+    def nt(self) -> int:
+        return len(self._ts)
+
+    def create_solver(self) -> "Solver":
+        return Solver()
+
+    def workspace(self, nbytes: int, device) -> torch.Tensor:
+        ws = self._ws.get(device)
+        if ws is None or ws.numel() < nbytes:
+            ws = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=device)
+            self._ws[device] = ws
+        return ws
+
+
+@dataclass
+class Solver:
+    """CVODES-equivalent solver options with AMICI's defaults
+    (include/amici/solver.h:1869-2147)."""
+
+    rtol: float = 1e-8
+    atol: float = 1e-16
+    rtol_fsa: float = math.nan
+    atol_fsa: float = math.nan
+    rtolB: float = math.nan
+    atolB: float = math.nan
+    quad_rtol: float = 1e-8
+    quad_atol: float = 1e-12
+    maxsteps: int = 10000
+    maxstepsB: int = 0
+    sensi: SensitivityOrder = SensitivityOrder.none
+    sensi_meth: SensitivityMethod = SensitivityMethod.forward
+    max_traj: int = 0
+
+    def set_relative_tolerance(self, v): self.rtol = float(v)
+    def get_relative_tolerance(self): return self.rtol
+    def set_absolute_tolerance(self, v): self.atol = float(v)
+    def get_absolute_tolerance(self): return self.atol
+    def set_relative_tolerance_fsa(self, v): self.rtol_fsa = float(v)
+    def set_absolute_tolerance_fsa(self, v): self.atol_fsa = float(v)
+    def set_relative_tolerance_b(self, v): self.rtolB = float(v)
+    def set_absolute_tolerance_b(self, v): self.atolB = float(v)
+    def set_relative_tolerance_quadratures(self, v): self.quad_rtol = float(v)
+    def set_absolute_tolerance_quadratures(self, v): self.quad_atol = float(v)
+    def set_max_steps(self, v): self.maxsteps = int(v)
+    def get_max_steps(self): return self.maxsteps
+    def set_max_steps_backward_problem(self, v): self.maxstepsB = int(v)
+    def set_sensitivity_order(self, v): self.sensi = SensitivityOrder(int(v))
+    def get_sensitivity_order(self): return self.sensi
+    def set_sensitivity_method(self, v): self.sensi_meth = SensitivityMethod(int(v))
+    def get_sensitivity_method(self): return self.sensi_meth
+
+    def set_linear_solver(self, v):
+        if str(v).lower() not in ("dense", "linearsolver.dense", "1"):
+            raise ValueError("this backend uses the dense direct linear solver")
+
+    def effective_method(self) -> int:
+        if self.sensi == SensitivityOrder.none:
+            return _lib.SENSI_NONE
+        return int(self.sensi_meth)
+
+    def c_opts(self) -> _lib.SolverOpts:
+        return _lib.make_opts(rtol=self.rtol, atol=self.atol, sensi_meth=self.effective_method(),
+                              maxsteps=self.maxsteps, rtol_fsa=self.rtol_fsa,
+                              atol_fsa=self.atol_fsa, rtolB=self.rtolB, atolB=self.atolB,
+                              quad_rtol=self.quad_rtol, quad_atol=self.quad_atol,
+                              maxstepsB=self.maxstepsB, max_traj=self.max_traj)
+
+
+class ExpData:
+    """Per-condition inputs (include/amici/edata.h): time grid, measurements
+    [nt, ny] (NaN = missing), noise scales [nt, ny], fixed parameters and
+    optional free parameters (log10 scale)."""
+
+    def __init__(self, model: Model | None = None):
+        self.ny = model.ny if model else 0
+        self.timepoints = model.get_timepoints() if model else np.zeros(0)
+        self.fixed_parameters = model.get_fixed_parameters() if model else np.zeros(0)
+        self.free_parameters = np.zeros(0)
+        self._meas = np.full((len(self.timepoints), self.ny), np.nan)
+        self._sigma = np.ones((len(self.timepoints), self.ny))
+        self.id = ""
+
+    def set_timepoints(self, ts):
+        self.timepoints = np.asarray(ts, np.float64).copy()
+        self._meas = np.full((len(self.timepoints), self.ny), np.nan)
+        self._sigma = np.ones((len(self.timepoints), self.ny))
+
+    def get_timepoints(self):
+        return self.timepoints.copy()
+
+    def set_measurements(self, m):
+        self._meas = np.asarray(m, np.float64).reshape(len(self.timepoints), self.ny).copy()
+
+    def get_measurements(self):
+        return self._meas.copy()
+
+    def set_noise_scales(self, s):
+        s = np.asarray(s, np.float64)
+        if s.ndim == 0:
+            s = np.full((len(self.timepoints), self.ny), float(s))
+        self._sigma = s.reshape(len(self.timepoints), self.ny).copy()
+
+    def get_noise_scales(self):
+        return self._sigma.copy()
+
+
+@dataclass
+class ReturnData:
+    """Results of one condition (fields and shapes of ReturnDataView)."""
+
+    ts: np.ndarray
+    x: np.ndarray | None = None
+    y: np.ndarray | None = None
+    sigmay: np.ndarray | None = None
+    sx: np.ndarray | None = None
+    sy: np.ndarray | None = None
+    llh: float = math.nan
+    sllh: np.ndarray | None = None
+    status: int = AMICI_NOT_RUN
+    numsteps: int = 0
+    numrhsevals: int = 0
+    numstepsB: int = 0
+    stats: np.ndarray = field(default_factory=lambda: np.zeros(_lib.AMX_NSTATS, np.int64))
+    messages: list = field(default_factory=list)
+
+    def __getitem__(self, key):
+        return getattr(self, key)
+
+
+def _as_dev(a, dev, dtype=torch.float64) -> torch.Tensor:
+    if isinstance(a, torch.Tensor):
+        return a.to(device=dev, dtype=dtype).contiguous()
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=dev)
+
+
+def _ptr(t):
+    return None if t is None else t.data_ptr()
+
+
+def simulate_batch(model: Model, solver: Solver, theta, kfix, ts, meas=None, sigma=None, *,
+                   want_x: bool = True, want_y: bool = True, want_sx: bool = False,
+                   want_stats: bool = True, sequential: bool = False, stream=None) -> dict:
+    """Simulates ``nb`` instances in one batch; arrays may be numpy or torch.
+
+    theta [nb, np] (log10), kfix [nb, nk], ts [nt] or [nb, nt], meas / sigma
+    [nb, nt, ny].  Returns a dict of device tensors: x, y, sx, llh, sllh,
+    status, stats.  ``sequential`` runs the plain one-instance-per-launch path.
+    """
+    dev = _device()
+    lib = model.lib
+    theta_t = _as_dev(theta, dev)
+    nb = theta_t.shape[0]
+    kfix_t = _as_dev(kfix if kfix is not None else np.zeros((nb, model.nk)), dev)
+    ts_t = _as_dev(ts, dev)
+    nt = ts_t.shape[-1]
+    ts_stride = nt if ts_t.dim() == 2 else 0
+    if sigma is None:
+        sigma = np.ones((nb, nt, model.ny))
+    sigma_t = _as_dev(sigma, dev)
+    meas_t = _as_dev(meas, dev) if meas is not None else None
+    if theta_t.shape != (nb, model.np) or kfix_t.shape != (nb, model.nk):
+        raise ValueError("theta / kfix shapes")
+    if sigma_t.shape != (nb, nt, model.ny) or (meas_t is not None and meas_t.shape != sigma_t.shape):
+        raise ValueError("meas / sigma shapes must be [nb, nt, ny]")
+    if ts_stride and ts_t.shape[0] != nb:
+        raise ValueError("ts must be [nt] or [nb, nt]")
+    meth = solver.effective_method()
+    opts = solver.c_opts()
+    f64 = dict(dtype=torch.float64, device=dev)
+    out = {
+        "x": torch.empty((nb, nt, model.nx_rdata), **f64) if want_x else None,
+        "y": torch.empty((nb, nt, model.ny), **f64) if want_y else None,
+        "sx": (torch.empty((nb, nt, model.np, model.nx_rdata), **f64)
+               if (want_sx and meth == _lib.SENSI_FORWARD) else None),
+        "llh": torch.empty(nb, **f64),
+        "sllh": torch.empty((nb, model.np), **f64) if meth != _lib.SENSI_NONE else None,
+        "status": torch.full((nb,), AMICI_NOT_RUN, dtype=torch.int32, device=dev),
+        "stats": (torch.zeros((nb, _lib.AMX_NSTATS), dtype=torch.int32, device=dev)
+                  if want_stats else None),
+    }
+    bin_ = _lib.BatchIn(nb, nt, ts_stride, _ptr(theta_t), _ptr(kfix_t) if model.nk else None,
+                        _ptr(ts_t), _ptr(meas_t), _ptr(sigma_t))
+    bout = _lib.BatchOut(*[_ptr(out[k]) for k in ("x", "y", "sx", "llh", "sllh", "status", "stats")])
+    nbytes = lib.workspace_bytes(opts, 1 if sequential else nb, nt)
+    ws = model.workspace(nbytes, dev)
+    st = stream if stream is not None else torch.cuda.current_stream(dev)
+    fn = lib.lib.amx_run_sequential if sequential else lib.lib.amx_run_batch
+    import ctypes
+    rc = fn(ctypes.byref(opts), ctypes.byref(bin_), ctypes.byref(bout), ws.data_ptr(), ws.numel(),
+            ctypes.c_void_p(st.cuda_stream))
+    if rc != 0:
+        raise RuntimeError(f"amx_run_batch returned {rc}")
+    out["_keepalive"] = (theta_t, kfix_t, ts_t, meas_t, sigma_t)
+    return out
+
+
+def run_simulations(model: Model, solver: Solver, edata_list: Sequence[ExpData],
+                    failfast: bool = True, num_threads: int = 1) -> list:
+    """All conditions of ``edata_list`` as one GPU batch; returns one
+    :class:`ReturnData` per condition.  With ``failfast`` every condition after
+    the first failed one is reported as not run (AMICI_NOT_RUN), as the serial
+    reference loop does (src/amici.cpp:246-260)."""
+    del num_threads
+    if len(edata_list) == 0:
+        return []
+    nb = len(edata_list)
+    nts = {len(e.timepoints) for e in edata_list}
+    if len(nts) != 1:
+        # group by time-grid length and reassemble in order
+        res = [None] * nb
+        for n in sorted(nts):
+            idx = [i for i, e in enumerate(edata_list) if len(e.timepoints) == n]
+            sub = run_simulations(model, solver, [edata_list[i] for i in idx], failfast=False)
+            for i, r in zip(idx, sub):
+                res[i] = r
+        return _apply_failfast(res, failfast)
+    theta = np.stack([e.free_parameters if len(e.free_parameters) else model.get_free_parameters()
+                      for e in edata_list])
+    kfix = np.stack([np.asarray(e.fixed_parameters, np.float64) if len(e.fixed_parameters)
+                     else model.get_fixed_parameters() for e in edata_list]).reshape(nb, model.nk)
+    ts = np.stack([e.timepoints for e in edata_list])
+    meas = np.stack([e.get_measurements() for e in edata_list])
+    sigma = np.stack([e.get_noise_scales() for e in edata_list])
+    if np.all(ts == ts[0:1]):
+        ts = ts[0]
+    meth = solver.effective_method()
+    out = simulate_batch(model, solver, theta, kfix, ts, meas, sigma,
+                         want_sx=(meth == _lib.SENSI_FORWARD))
+    torch.cuda.synchronize()
+    host = {k: (v.cpu().numpy() if isinstance(v, torch.Tensor) else None)
+            for k, v in out.items() if k != "_keepalive"}
+    C = model.network.observable_matrix
+    res = []
+    for b, e in enumerate(edata_list):
+        st = host["stats"][b]
+        rd = ReturnData(ts=np.asarray(e.timepoints), x=host["x"][b], y=host["y"][b],
+                        sigmay=sigma[b], llh=float(host["llh"][b]), status=int(host["status"][b]),
+                        numsteps=int(st[0]), numrhsevals=int(st[1]), numstepsB=int(st[8]),
+                        stats=st.astype(np.int64))
+        if host["sllh"] is not None:
+            rd.sllh = host["sllh"][b]
+        if host["sx"] is not None:
+            rd.sx = host["sx"][b]
+            rd.sy = np.einsum("tpx,yx->tpy", rd.sx, C)
+        if rd.status != AMICI_SUCCESS:
+            rd.messages.append(model.lib.status_string(rd.status))
+        res.append(rd)
+    return _apply_failfast(res, failfast)
+
+
+def _apply_failfast(res: list, failfast: bool) -> list:
+    if not failfast:
+        return res
+    failed = False
+    for i, r in enumerate(res):
+        if failed:
+            res[i] = ReturnData(ts=r.ts, status=AMICI_NOT_RUN)
+        elif r.status != AMICI_SUCCESS:
+            failed = True
+    return res
+
+
+def run_simulation(model: Model, solver: Solver, edata: ExpData | None = None) -> ReturnData:
+    """Single condition (``amici.sim.sundials.run_simulation``)."""
+    if edata is None:
+        edata = ExpData(model)
+    return run_simulations(model, solver, [edata], failfast=False)[0]
+
+
+runAmiciSimulations = run_simulations
+runAmiciSimulation = run_simulation

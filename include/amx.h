@@ -1,0 +1,93 @@
+/*
+ * amx — C ABI of the MI355X (gfx950) batched stiff-ODE simulation and
+ * sensitivity backend.  Every generated model module libamx_<model>.so exports
+ * exactly these symbols.  All pointers in amx_batch_in / amx_batch_out and the
+ * workspace are device pointers; nothing crosses this boundary as an
+ * exception.
+ *
+ * Reference interfaces replaced:
+ *   amx_run_batch      amici::run_simulations(Solver const&, std::vector<ExpData*> const&,
+ *                        Model const&, bool failfast, int num_threads)
+ *                        (include/amici/amici.h:36-39, src/amici.cpp:220-264) for
+ *                        ODE models with sensitivity method none / forward / adjoint,
+ *                        returning per-condition x, y, sx, llh, sllh and status
+ *                        (ReturnData, include/amici/rdata.h:195-245).
+ *   amx_solver_opts    the Solver options used on this path, with AMICI's defaults
+ *                        (include/amici/solver.h:1869-2147).
+ *   amx_model_info     model dimensions (ModelDimensions, include/amici/model_dimensions.h:59-156).
+ *   amx_status_string  simulation_status_to_str_map (include/amici/amici.h).
+ */
+#ifndef AMX_H
+#define AMX_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AMX_ABI_VERSION 1
+#define AMX_NSTATS 16
+
+/* sensitivity method (amici::SensitivityMethod, include/amici/defines.h) */
+#define AMX_SENSI_NONE 0
+#define AMX_SENSI_FORWARD 1
+#define AMX_SENSI_ADJOINT 2
+
+typedef struct {
+    double rtol, atol;         /* state tolerances (defaults 1e-8 / 1e-16) */
+    double rtol_fsa, atol_fsa; /* forward sensitivities; NaN -> rtol / atol */
+    double rtolB, atolB;       /* adjoint states; NaN -> rtol / atol */
+    double quad_rtol, quad_atol; /* adjoint quadratures (defaults 1e-8 / 1e-12) */
+    int maxsteps;              /* forward step limit (default 10000) */
+    int maxstepsB;             /* backward step limit per interval; 0 -> 100 * maxsteps */
+    int sensi_meth;            /* AMX_SENSI_* */
+    int max_traj;              /* adjoint: stored forward steps per instance (0 -> maxsteps + 1) */
+} amx_solver_opts;
+
+typedef struct {
+    int nb;                /* instances (conditions / parameter sets)                 */
+    int nt;                /* output times per instance                               */
+    int ts_stride;         /* 0: ts shared by all instances, nt: one row per instance */
+    const double* theta;   /* [nb][np] free parameters, log10 scale                   */
+    const double* kfix;    /* [nb][nk] fixed parameters                               */
+    const double* ts;      /* [1 or nb][nt] ascending, >= t0 = 0                      */
+    const double* meas;    /* [nb][nt][ny] measurements, NaN = missing; may be NULL   */
+    const double* sigma;   /* [nb][nt][ny] noise standard deviations                  */
+} amx_batch_in;
+
+typedef struct {
+    double* x;      /* [nb][nt][nx] or NULL                         */
+    double* y;      /* [nb][nt][ny] or NULL                         */
+    double* sx;     /* [nb][nt][np][nx] or NULL (forward only)      */
+    double* llh;    /* [nb] log-likelihood (required)               */
+    double* sllh;   /* [nb][np] gradient of llh w.r.t. log10 params */
+    int* status;    /* [nb] AMICI status code per instance (required) */
+    int* stats;     /* [nb][AMX_NSTATS] solver counters or NULL      */
+} amx_batch_out;
+
+/* dims[0..7] = nx, nr, np, nk, ny, team lanes per instance, noise model, ABI version */
+int amx_model_info(int* dims);
+
+/* Bytes of device workspace amx_run_batch needs for this batch / options. */
+size_t amx_workspace_bytes(const amx_solver_opts* opts, int nb, int nt);
+
+/* Runs the whole batch on `stream` (a hipStream_t; NULL = default stream).
+ * Returns 0 when the launches were issued; per-instance results in
+ * out->status after the stream completes.  Error codes:
+ *   -22 invalid argument, -23 workspace too small, -30 HIP launch error. */
+int amx_run_batch(const amx_solver_opts* opts, const amx_batch_in* in, amx_batch_out* out,
+                  void* workspace, size_t workspace_bytes, void* stream);
+
+/* Plain GPU BDF baseline: the same integrator, one instance per kernel launch
+ * (sequential launches on `stream`), forward sensitivities or none. */
+int amx_run_sequential(const amx_solver_opts* opts, const amx_batch_in* in, amx_batch_out* out,
+                       void* workspace, size_t workspace_bytes, void* stream);
+
+/* Text for an AMICI status code. */
+const char* amx_status_string(int status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

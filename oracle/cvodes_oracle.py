@@ -1,0 +1,145 @@
+"""TEST INFRASTRUCTURE ONLY — Python binding of the oracle library.
+
+The oracle is the reference's own SUNDIALS CVODES 7.1.1 (compiled from
+/root/reference/ThirdParty/sundials by oracle/Makefile into
+oracle/_ref/libamx_oracle.so) driven by oracle/cvodes_harness.c, a C
+restatement of AMICI's forward / backward problem, objective and chain-rule
+layer.  Only tests, ``__graft_entry__.smoke()`` and the cpu_baseline leg of
+``bench.py`` may use it, and only as the checker or the reported CPU baseline.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "_ref" / "libamx_oracle.so"
+NCOUNTERS = 16
+CV_SIMULTANEOUS, CV_STAGGERED = 1, 2
+CV_HERMITE, CV_POLYNOMIAL = 1, 2
+
+
+class _Model(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("nr", ctypes.c_int), ("np", ctypes.c_int),
+                ("nk", ctypes.c_int), ("ny", ctypes.c_int),
+                ("rptr", ctypes.c_void_p), ("ridx", ctypes.c_void_p),
+                ("pptr", ctypes.c_void_p), ("pidx", ctypes.c_void_p),
+                ("kidx", ctypes.c_void_p), ("x0c", ctypes.c_void_p),
+                ("x0k", ctypes.c_void_p), ("C", ctypes.c_void_p), ("noise", ctypes.c_int)]
+
+
+class _Opts(ctypes.Structure):
+    _fields_ = [("rtol", ctypes.c_double), ("atol", ctypes.c_double),
+                ("quad_rtol", ctypes.c_double), ("quad_atol", ctypes.c_double),
+                ("maxsteps", ctypes.c_long), ("sensi", ctypes.c_int), ("ism", ctypes.c_int),
+                ("interp", ctypes.c_int), ("stldet", ctypes.c_int)]
+
+
+def build(reference: str = "/root/reference/ThirdParty/sundials") -> Path:
+    """Compiles the oracle library (needs the reference sources)."""
+    subprocess.run(["make", "-s", "-C", str(HERE), f"REF={reference}", "-j8"], check=True)
+    return LIB
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            if Path("/root/reference/ThirdParty/sundials").exists():
+                build()
+            else:
+                raise FileNotFoundError(f"{LIB} missing and the reference is not available")
+        L = ctypes.CDLL(str(LIB))
+        L.amx_oracle_simulate_batch.restype = ctypes.c_int
+        L.amx_oracle_Jy.restype = ctypes.c_double
+        L.amx_oracle_Jy.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        L.amx_oracle_dJydy.restype = ctypes.c_double
+        L.amx_oracle_dJydy.argtypes = L.amx_oracle_Jy.argtypes
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+class OracleModel:
+    def __init__(self, network):
+        self.net = network
+        t = network.flat_tables()
+        self._keep = t
+        self.c = _Model(network.nx, network.nr, network.np, network.nk, network.ny,
+                        _p(t["rptr"]), _p(t["ridx"]), _p(t["pptr"]), _p(t["pidx"]),
+                        _p(t["kidx"]), _p(t["x0c"]), _p(t["x0k"]), _p(t["C"]),
+                        int(network.noise))
+
+    def xdot(self, theta, x):
+        p = 10.0 ** np.asarray(theta, np.float64)
+        x = np.ascontiguousarray(x, np.float64)
+        out = np.zeros(self.net.nx)
+        lib().amx_oracle_xdot(ctypes.byref(self.c), _p(p), _p(x), _p(out))
+        return out
+
+    def jac(self, theta, x):
+        p = 10.0 ** np.asarray(theta, np.float64)
+        x = np.ascontiguousarray(x, np.float64)
+        out = np.zeros((self.net.nx, self.net.nx))
+        lib().amx_oracle_jac(ctypes.byref(self.c), _p(p), _p(x), _p(out))
+        return out
+
+    def dxdotdp(self, theta, x):
+        p = 10.0 ** np.asarray(theta, np.float64)
+        x = np.ascontiguousarray(x, np.float64)
+        out = np.zeros((self.net.nx, self.net.np))
+        lib().amx_oracle_dxdotdp(ctypes.byref(self.c), _p(p), _p(x), _p(out))
+        return out
+
+
+def simulate(network, theta, kfix, ts, meas=None, sigma=None, *, rtol=1e-10, atol=1e-14,
+             sensi=0, ism=CV_SIMULTANEOUS, interp=CV_POLYNOMIAL, stldet=1, maxsteps=100000,
+             quad_rtol=1e-8, quad_atol=1e-12, nthreads=1, want_sx=False):
+    """CVODES-oracle batch: theta [nb, np], kfix [nb, nk], ts [nt] or [nb, nt],
+    meas / sigma [nb, nt, ny].  sensi: 0 none, 1 forward, 2 adjoint.
+    Returns dict of numpy arrays x, y, sx, llh, sllh, status, counters."""
+    om = OracleModel(network)
+    theta = np.ascontiguousarray(np.atleast_2d(theta), np.float64)
+    nb = theta.shape[0]
+    kfix = np.ascontiguousarray(np.asarray(kfix, np.float64).reshape(nb, network.nk))
+    ts = np.asarray(ts, np.float64)
+    nt = ts.shape[-1]
+    ts = np.ascontiguousarray(np.broadcast_to(ts, (nb, nt)))
+    if sigma is None:
+        sigma = np.ones((nb, nt, network.ny))
+    sigma = np.ascontiguousarray(np.broadcast_to(sigma, (nb, nt, network.ny)), np.float64)
+    meas = (np.ascontiguousarray(np.broadcast_to(meas, (nb, nt, network.ny)), np.float64)
+            if meas is not None else None)
+    x = np.zeros((nb, nt, network.nx))
+    y = np.zeros((nb, nt, network.ny))
+    sx = np.zeros((nb, nt, network.np, network.nx)) if (want_sx and sensi == 1) else None
+    llh = np.zeros(nb)
+    sllh = np.zeros((nb, network.np))
+    counters = np.zeros((nb, NCOUNTERS), np.int64)
+    status = np.zeros(nb, np.int32)
+    opts = _Opts(rtol, atol, quad_rtol, quad_atol, maxsteps, sensi, ism, interp, stldet)
+    lib().amx_oracle_simulate_batch(
+        ctypes.byref(om.c), ctypes.byref(opts), ctypes.c_int(nb), _p(theta), _p(kfix),
+        ctypes.c_int(nt), _p(ts), _p(meas), _p(sigma), _p(x), _p(y), _p(sx), _p(llh), _p(sllh),
+        _p(counters), _p(status), ctypes.c_int(nthreads))
+    return dict(x=x, y=y, sx=sx, llh=llh, sllh=sllh if sensi else None, status=status,
+                counters=counters)
+
+
+def Jy(noise: int, y: float, sigma: float, m: float) -> float:
+    return lib().amx_oracle_Jy(noise, y, sigma, m)
+
+
+def dJydy(noise: int, y: float, sigma: float, m: float) -> float:
+    return lib().amx_oracle_dJydy(noise, y, sigma, m)
