@@ -14,7 +14,7 @@ AMX_NSTATS = 16
 SENSI_NONE, SENSI_FORWARD, SENSI_ADJOINT = 0, 1, 2
 
 SYMBOLS = ("amx_model_info", "amx_workspace_bytes", "amx_run_batch", "amx_run_sequential",
-           "amx_status_string")
+           "amx_set_trace", "amx_status_string")
 
 
 class SolverOpts(ctypes.Structure):
@@ -25,6 +25,7 @@ class SolverOpts(ctypes.Structure):
         ("quad_rtol", ctypes.c_double), ("quad_atol", ctypes.c_double),
         ("maxsteps", ctypes.c_int), ("maxstepsB", ctypes.c_int),
         ("sensi_meth", ctypes.c_int), ("max_traj", ctypes.c_int),
+        ("maxtime", ctypes.c_double),
     ]
 
 
@@ -68,6 +69,8 @@ class ModelLib:
                            ctypes.POINTER(BatchOut), ctypes.c_void_p, ctypes.c_size_t,
                            ctypes.c_void_p]
             fn.restype = ctypes.c_int
+        L.amx_set_trace.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        L.amx_set_trace.restype = ctypes.c_int
         L.amx_status_string.argtypes = [ctypes.c_int]
         L.amx_status_string.restype = ctypes.c_char_p
         dims = (ctypes.c_int * 8)()
@@ -86,6 +89,7 @@ class ModelLib:
 
 def make_opts(rtol=1e-8, atol=1e-16, sensi_meth=SENSI_NONE, maxsteps=10000, rtol_fsa=math.nan,
               atol_fsa=math.nan, rtolB=math.nan, atolB=math.nan, quad_rtol=1e-8,
-              quad_atol=1e-12, maxstepsB=0, max_traj=0) -> SolverOpts:
+              quad_atol=1e-12, maxstepsB=0, max_traj=0, maxtime=0.0) -> SolverOpts:
     return SolverOpts(rtol, atol, rtol_fsa, atol_fsa, rtolB, atolB, quad_rtol, quad_atol,
-                      int(maxsteps), int(maxstepsB), int(sensi_meth), int(max_traj))
+                      int(maxsteps), int(maxstepsB), int(sensi_meth), int(max_traj),
+                      float(maxtime))

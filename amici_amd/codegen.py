@@ -38,10 +38,10 @@ HIP_FLAGS = [
 
 
 def default_team(nx: int) -> int:
-    t = 1
-    while t < nx:
-        t *= 2
-    return max(t, 1)
+    """One instance per 64-lane wavefront (team = wavefront) for nx <= 64."""
+    if nx > 64:
+        raise ValueError("models with more than 64 states need the workgroup-per-instance engine")
+    return 64
 
 
 def _csr(rows: list, n: int):
@@ -143,6 +143,12 @@ def _fmt_array(ctype: str, name: str, vals) -> str:
 
 
 def emit_model_source(net: ReactionNetwork, team: int | None = None) -> str:
+    """Model module source: traits plus the C ABI (amx_capi.inc)."""
+    return emit_model_traits(net, team) + '#include "amx_capi.inc"\n'
+
+
+def emit_model_traits(net: ReactionNetwork, team: int | None = None) -> str:
+    """The AmxModel traits struct (dimensions, team size, sparsity tables)."""
     t = model_tables(net)
     team = team or default_team(net.nx)
     if team < net.nx or team > 64:
@@ -164,7 +170,6 @@ def emit_model_source(net: ReactionNetwork, team: int | None = None) -> str:
     for name in _DBL_TABLES:
         lines.append(_fmt_array("double", name, t[name]))
     lines.append("};\n\n")
-    lines.append('#include "amx_capi.inc"\n')
     return "".join(lines)
 
 

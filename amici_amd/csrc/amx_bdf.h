@@ -49,6 +49,7 @@ enum : int {
   AMX_RHSFUNC_FAIL = -8,
   AMX_ILL_INPUT = -22,
   AMX_TOO_CLOSE = -27,
+  AMX_MAX_TIME_EXCEEDED = -1000,
   AMX_NOT_RUN = -1001,
   AMX_TRAJ_FULL = -1010,  // forward-trajectory buffer of an instance exhausted
 };
@@ -61,23 +62,13 @@ enum : int { NO_FAILURES = 0, FAIL_BAD_J = 1, FAIL_OTHER = 2 };
 
 __device__ __forceinline__ double powr(double b, double e) { return pow(b, e); }
 
-// Hides a value from the optimizer so that a select between array elements is
-// not rewritten into a load through a selected address (which would force the
-// whole register-resident integrator state into scratch memory).
-__device__ __forceinline__ double opaque(double v) {
-  asm volatile("" : "+v"(v));
-  return v;
-}
-
-// a[i] for a runtime index i, without dynamic indexing.
+// a[i] for a runtime index i
 template <int N>
-__device__ __forceinline__ double sel(const double (&a)[N], int i) {
-  double r = opaque(a[0]);
+__device__ double sel(const double (&a)[N], int i) {
+  double r = a[0];
 #pragma unroll
-  for (int k = 1; k < N; ++k) {
-    const double v = opaque(a[k]);
-    r = (i == k) ? v : r;
-  }
+  for (int k = 1; k < N; ++k)
+    if (i == k) r = a[k];
   return r;
 }
 
@@ -90,7 +81,7 @@ struct LURows {
   int src;       // lane that holds pivot row number `lane`
   int perm[NX];  // perm[k] = lane of the k-th pivot row (team-uniform)
 
-  __device__ __forceinline__ bool factor(const Team<T>& tm) {
+  __device__ bool factor(const Team<T>& tm) {
     pos = (tm.lane < NX) ? -1 : NX;
     src = 0;
     dinv = 0.0;
@@ -118,7 +109,7 @@ struct LURows {
   }
 
   // Solves (LU) x = b; b is this lane's component of the right-hand side.
-  __device__ __forceinline__ double solve(double b, const Team<T>& tm) const {
+  __device__ double solve(double b, const Team<T>& tm) const {
 #pragma unroll
     for (int k = 0; k < NX; ++k) {
       const double yk = tm.bcast(b, perm[k]);
@@ -134,12 +125,15 @@ struct LURows {
   }
 };
 
+constexpr int TRACE_REC = 8;
+
 struct SolverParams {
   double rtol, atol;
   double rtolQ, atolQ;
   double rtolS, atolS;
   int maxsteps;
   int errconQ;
+  unsigned long long deadline;  // wall_clock64() limit, 0 = none
 };
 
 // BDF integrator over a Problem P.  P provides:
@@ -158,7 +152,7 @@ struct BDF {
   static constexpr bool QUAD = NQ > 0;
   static constexpr bool SENS = P::NS > 0;
 
-  P pb;
+  P& pb;
   Team<T> tm;
   SolverParams sp;
   bool active;  // lane < NX
@@ -178,17 +172,34 @@ struct BDF {
   bool jcur, tstopset;
   // counters
   int nfe, nsetups, nje, nni, netf, ncfn, nfQe, nniS;
+  // optional step trace (team lane 0 writes records of TRACE_REC doubles)
+  double* trace;
+  int trace_cap, trace_n;
 
-  __device__ __forceinline__ BDF(const P& p, const SolverParams& s) : pb(p), sp(s) {
+  __device__ BDF(P& p, const SolverParams& s) : pb(p), sp(s) {
     active = tm.lane < NX;
+    trace = nullptr;
+    trace_cap = 0;
+    trace_n = 0;
+  }
+
+  // event: 1 accepted step, 2 nonlinear-solver failure, 3 error-test failure
+  __device__ void record(int event, double v) {
+    if (trace == nullptr || trace_n >= trace_cap) return;
+    if (tm.lane == 0) {
+      double* r = trace + (size_t)trace_n * TRACE_REC;
+      r[0] = event; r[1] = tn; r[2] = h; r[3] = q; r[4] = v;
+      r[5] = nni; r[6] = nsetups; r[7] = nje;
+    }
+    trace_n++;
   }
 
   // ---------------- vector helpers ----------------
-  __device__ __forceinline__ double wrms(double v, double w) const {
+  __device__ double wrms(double v, double w) const {
     const double e = active ? v * w : 0.0;
     return sqrt(tm.sum(e * e) / NX);
   }
-  __device__ __forceinline__ double wrmsQ(const double (&v)[EPQ], const double (&w)[EPQ]) const {
+  __device__ double wrmsQ(const double (&v)[EPQ], const double (&w)[EPQ]) const {
     double acc = 0.0;
 #pragma unroll
     for (int k = 0; k < EPQ; ++k) {
@@ -198,19 +209,19 @@ struct BDF {
     }
     return sqrt(tm.sum(acc) / (NQ > 0 ? NQ : 1));
   }
-  __device__ __forceinline__ void set_ewt() { ewt = active ? 1.0 / (sp.rtol * fabs(zn[0]) + sp.atol) : 0.0; }
-  __device__ __forceinline__ void set_ewtQ() {
+  __device__ void set_ewt() { ewt = active ? 1.0 / (sp.rtol * fabs(zn[0]) + sp.atol) : 0.0; }
+  __device__ void set_ewtQ() {
 #pragma unroll
     for (int k = 0; k < EPQ; ++k) {
       const int g = tm.lane + k * T;
       ewtQ[k] = (g < NQ) ? 1.0 / (sp.rtolQ * fabs(znQ[0][k]) + sp.atolQ) : 0.0;
     }
   }
-  __device__ __forceinline__ bool quad_errcon() const { return QUAD && sp.errconQ; }
+  __device__ bool quad_errcon() const { return QUAD && sp.errconQ; }
 
   // ---------------- initialisation ----------------
   // CVodeInit state (cvodes.c:677-886) for y0 on this lane, quadratures zero.
-  __device__ __forceinline__ void init(double t0, double y0, const double* q0) {
+  __device__ void init(double t0, double y0, const double* q0) {
     tn = t0;
 #pragma unroll
     for (int j = 0; j < LMAX; ++j) zn[j] = 0.0;
@@ -241,12 +252,12 @@ struct BDF {
 
   // ---------------- Nordsieck array manipulation ----------------
   template <class F>
-  __device__ __forceinline__ void for_zn(F&& f) {  // f(j, double& znj_state, double (&znQj)[EPQ])
+  __device__ void for_zn(F&& f) {  // f(j, double& znj_state, double (&znQj)[EPQ])
 #pragma unroll
     for (int j = 0; j < LMAX; ++j) f(j, zn[j], znQ[j]);
   }
 
-  __device__ __forceinline__ void predict() {
+  __device__ void predict() {
     tn += h;
     if (tstopset && (tn - tstop) * h > 0.0) tn = tstop;
 #pragma unroll
@@ -262,7 +273,7 @@ struct BDF {
     if constexpr (SENS) pb.sens_predict(q);
   }
 
-  __device__ __forceinline__ void restore(double saved_t) {
+  __device__ void restore(double saved_t) {
     tn = saved_t;
 #pragma unroll
     for (int k = 1; k <= QMAX; ++k)
@@ -277,7 +288,7 @@ struct BDF {
     if constexpr (SENS) pb.sens_restore(q);
   }
 
-  __device__ __forceinline__ void rescale() {
+  __device__ void rescale() {
     double c = eta;
 #pragma unroll
     for (int j = 1; j <= QMAX; ++j) {
@@ -296,7 +307,7 @@ struct BDF {
   }
 
   // Adds coef[j] * zn[src] to zn[j] for 2 <= j <= jmax.
-  __device__ __forceinline__ void increase_bdf() {
+  __device__ void increase_bdf() {
     double ll[LMAX];
 #pragma unroll
     for (int i = 0; i < LMAX; ++i) ll[i] = 0.0;
@@ -323,7 +334,8 @@ struct BDF {
     for (int j = 2; j <= QMAX; ++j)
       if (j <= q) zn[j] = fma(ll[j], znL, zn[j]);
 #pragma unroll
-    for (int j = 2; j <= QMAX; ++j) zn[j] = (j == q + 1) ? opaque(znL) : zn[j];
+    for (int j = 2; j <= QMAX; ++j)
+      if (j == q + 1) zn[j] = znL;
     if constexpr (QUAD) {
 #pragma unroll
       for (int e = 0; e < EPQ; ++e) {
@@ -332,13 +344,14 @@ struct BDF {
         for (int j = 2; j <= QMAX; ++j)
           if (j <= q) znQ[j][e] = fma(ll[j], zQL, znQ[j][e]);
 #pragma unroll
-        for (int j = 2; j <= QMAX; ++j) znQ[j][e] = (j == q + 1) ? opaque(zQL) : znQ[j][e];
+        for (int j = 2; j <= QMAX; ++j)
+          if (j == q + 1) znQ[j][e] = zQL;
       }
     }
     if constexpr (SENS) pb.sens_increase(q, A1, ll);
   }
 
-  __device__ __forceinline__ void decrease_bdf() {
+  __device__ void decrease_bdf() {
     double ll[LMAX];
 #pragma unroll
     for (int i = 0; i < LMAX; ++i) ll[i] = 0.0;
@@ -361,12 +374,10 @@ struct BDF {
       if constexpr (QUAD) {
 #pragma unroll
         for (int e = 0; e < EPQ; ++e) {
-          double zq = opaque(znQ[0][e]);
+          double zq = znQ[0][e];
 #pragma unroll
-          for (int j = 1; j < LMAX; ++j) {
-            const double v = opaque(znQ[j][e]);
-            zq = (j == q) ? v : zq;
-          }
+          for (int j = 1; j < LMAX; ++j)
+            if (j == q) zq = znQ[j][e];
 #pragma unroll
           for (int j = 2; j < QMAX; ++j)
             if (j < q) znQ[j][e] = fma(-ll[j], zq, znQ[j][e]);
@@ -376,13 +387,13 @@ struct BDF {
     }
   }
 
-  __device__ __forceinline__ void adjust_order(int deltaq) {
+  __device__ void adjust_order(int deltaq) {
     if (q == 2 && deltaq != 1) return;
     if (deltaq == 1) increase_bdf();
     else if (deltaq == -1) decrease_bdf();
   }
 
-  __device__ __forceinline__ void adjust_params() {
+  __device__ void adjust_params() {
     if (qprime != q) {
       adjust_order(qprime - q);
       q = qprime;
@@ -393,7 +404,7 @@ struct BDF {
   }
 
   // ---------------- method coefficients ----------------
-  __device__ __forceinline__ void set_bdf() {
+  __device__ void set_bdf() {
     double xi_inv = 1.0, xistar_inv = 1.0, alpha0 = -1.0, alpha0_hat = -1.0;
     double hsum = h;
 #pragma unroll
@@ -444,7 +455,7 @@ struct BDF {
     tq[4] = CORTES / tq[2];
   }
 
-  __device__ __forceinline__ void set_coeffs() {
+  __device__ void set_coeffs() {
     set_bdf();
     rl1 = 1.0 / l[1];
     gamma = h * rl1;
@@ -454,7 +465,7 @@ struct BDF {
 
   // ---------------- linear system ----------------
   // cvLsSetup + cvNlsLSetup; returns NLS_OK or NLS_RECOVER (singular).
-  __device__ __forceinline__ int lsetup(int cf) {
+  __device__ int lsetup(int cf) {
     const double dgamma = fabs(gamma / gammap - 1.0);
     const bool jbad = (nst == 0) || (nst >= nstlj + MSBJ) ||
                       (cf == FAIL_BAD_J && dgamma < DGMAX_JBAD) || (cf == FAIL_OTHER);
@@ -477,7 +488,7 @@ struct BDF {
     return ok ? NLS_OK : NLS_RECOVER;
   }
 
-  __device__ __forceinline__ double lsolve(double b) const {
+  __device__ double lsolve(double b) const {
     double x = lu.solve(b, tm);
     if (gamrat != 1.0) x *= 2.0 / (1.0 + gamrat);
     return x;
@@ -485,7 +496,7 @@ struct BDF {
 
   // ---------------- nonlinear solve (Newton) ----------------
   // cvNlsResidual: y = zn0 + acor, ftemp = f(tn, y), returns residual.
-  __device__ __forceinline__ double residual() {
+  __device__ double residual() {
     y = zn[0] + acor;
     ftemp = pb.rhs(tn, y);
     nfe++;
@@ -493,9 +504,9 @@ struct BDF {
   }
 
   // SUNNonlinSolSolve_Newton with cvNlsConvTest; returns NLS_OK / NLS_RECOVER / <0.
-  __device__ __forceinline__ int newton(bool callSetup) {
+  __device__ int newton(bool callSetup) {
     bool jbad = false;
-    for (;;) {
+    for (int attempt = 0;; ++attempt) {
       int m = 0;
       double res = residual();
       int ret = NLS_OK;
@@ -507,7 +518,8 @@ struct BDF {
       if (ret == NLS_OK) {
         for (;;) {
           nni++;
-          const double delta = lsolve(-res);
+          double delta = lsolve(-res);
+          if (!active) delta = 0.0;
           acor += delta;
           const double del = wrms(delta, ewt);
           if (m > 0) crate = fmax(CRDOWN * crate, del / delp);
@@ -525,7 +537,9 @@ struct BDF {
           if (!isfinite(tm.max(active ? fabs(res) : 0.0))) { ret = NLS_RECOVER; break; }
         }
       }
-      if (ret == NLS_RECOVER && !jcur) {
+      // (CVODES retries with a fresh Jacobian while it is not current; the
+      // attempt cap only guards against non-finite step data)
+      if (ret == NLS_RECOVER && !jcur && attempt < 3) {
         callSetup = true;
         jbad = true;
         acor = 0.0;
@@ -535,7 +549,7 @@ struct BDF {
     }
   }
 
-  __device__ __forceinline__ int nls(int nflag) {
+  __device__ int nls(int nflag) {
     convfail = (nflag == FIRST_CALL || nflag == PREV_ERR_FAIL) ? NO_FAILURES : FAIL_OTHER;
     const bool callSetup = (nflag == PREV_CONV_FAIL) || (nflag == PREV_ERR_FAIL) ||
                            (nst == 0) || (nst >= nstlp + MSBP) ||
@@ -549,8 +563,9 @@ struct BDF {
   }
 
   // cvHandleNFlag
-  __device__ __forceinline__ int handle_nflag(int& nflag, double saved_t, int& ncf) {
+  __device__ int handle_nflag(int& nflag, double saved_t, int& ncf) {
     if (nflag == NLS_OK) return DO_ERROR_TEST;
+    record(2, (double)nflag);
     ncfn++;
     restore(saved_t);
     ncf++;
@@ -564,10 +579,11 @@ struct BDF {
 
   // cvDoErrorTest (state part); extra(...) re-evaluates quadrature / sensitivity
   // derivatives on the order-1 restart branch.
-  __device__ __forceinline__ int error_test(int& nflag, double saved_t, double acor_nrm, int& nef) {
+  __device__ int error_test(int& nflag, double saved_t, double acor_nrm, int& nef) {
     const double dsm = acor_nrm * tq[2];
     dsm_test = dsm;
     if (dsm <= 1.0) return AMX_SUCCESS;
+    record(3, dsm);
     nef++;
     netf++;
     nflag = PREV_ERR_FAIL;
@@ -610,7 +626,7 @@ struct BDF {
   }
 
   // cvQuadNls
-  __device__ __forceinline__ void quad_correct() {
+  __device__ void quad_correct() {
     double fq[EPQ];
     pb.quad_rhs(tn, y, fq);
     nfQe++;
@@ -622,7 +638,7 @@ struct BDF {
   }
 
   // cvCompleteStep
-  __device__ __forceinline__ void complete_step() {
+  __device__ void complete_step() {
     nst++;
     nscon++;
     hu = h;
@@ -651,7 +667,7 @@ struct BDF {
     }
   }
 
-  __device__ __forceinline__ void set_eta() {
+  __device__ void set_eta() {
     if (eta > 0.0 && eta < ETA_MAX_FX) {
       eta = 1.0;
       hprime = h;
@@ -666,7 +682,7 @@ struct BDF {
     }
   }
 
-  __device__ __forceinline__ double norm_with_extras(double nstate, const double (&vQ)[EPQ], bool useQ,
+  __device__ double norm_with_extras(double nstate, const double (&vQ)[EPQ], bool useQ,
                                      double sensnorm, bool useS) const {
     double r = nstate;
     if constexpr (QUAD)
@@ -675,18 +691,16 @@ struct BDF {
     return r;
   }
 
-  __device__ __forceinline__ double compute_etaqm1() {
+  __device__ double compute_etaqm1() {
     etaqm1 = 0.0;
     if (q > 1) {
       double zq[EPQ];
 #pragma unroll
       for (int e = 0; e < EPQ; ++e) {
-        zq[e] = opaque(znQ[0][e]);
+        zq[e] = znQ[0][e];
 #pragma unroll
-        for (int j = 1; j < LMAX; ++j) {
-          const double v = opaque(znQ[j][e]);
-          zq[e] = (j == q) ? v : zq[e];
-        }
+        for (int j = 1; j < LMAX; ++j)
+          if (j == q) zq[e] = znQ[j][e];
       }
       double ddn = wrms(sel(zn, q), ewt);
       double sn = 0.0;
@@ -697,7 +711,7 @@ struct BDF {
     return etaqm1;
   }
 
-  __device__ __forceinline__ double compute_etaqp1() {
+  __device__ double compute_etaqp1() {
     etaqp1 = 0.0;
     if (q != QMAX) {
       if (saved_tq5 == 0.0) return etaqp1;
@@ -715,7 +729,7 @@ struct BDF {
     return etaqp1;
   }
 
-  __device__ __forceinline__ void choose_eta() {
+  __device__ void choose_eta() {
     const double etam = fmax(etaqm1, fmax(etaq, etaqp1));
     if (etam > 0.0 && etam < ETA_MAX_FX) {
       eta = 1.0;
@@ -738,7 +752,7 @@ struct BDF {
     }
   }
 
-  __device__ __forceinline__ void prepare_next_step(double dsm) {
+  __device__ void prepare_next_step(double dsm) {
     if (etamax == 1.0) {
       qwait = max(qwait, 2);
       qprime = q;
@@ -761,13 +775,14 @@ struct BDF {
   }
 
   // ---------------- one step: cvStep ----------------
-  __device__ __forceinline__ int step() {
+  __device__ int step() {
     int ncf = 0, nef = 0, nefQ = 0, ncfS = 0, nefS = 0;
     if (nst > 0 && hprime != h) adjust_params();
     const double saved_t = tn;
     int nflag = FIRST_CALL;
     double dsm = 0.0;
-    for (;;) {
+    for (int attempt = 0;; ++attempt) {
+      if (attempt >= MXNCF + MXNEF + 8) return AMX_CONV_FAILURE;
       predict();
       set_coeffs();
       nflag = nls(nflag);
@@ -807,6 +822,7 @@ struct BDF {
       break;
     }
     complete_step();
+    record(1, dsm);
     prepare_next_step(dsm);
     etamax = (nst <= SMALL_NST) ? ETA_MAX_ES : ETA_MAX_GS;
     dsm_last = dsm;
@@ -814,7 +830,7 @@ struct BDF {
   }
 
   // ---------------- initial step size: cvHin ----------------
-  __device__ __forceinline__ double upper_bound_h0(double tdist) {
+  __device__ double upper_bound_h0(double tdist) {
     double t1 = active ? (HUB_FACTOR * fabs(zn[0]) + 1.0 / ewt) : 1.0;
     double hub_inv = tm.max(active ? fabs(zn[1]) / t1 : 0.0);
     if constexpr (QUAD) {
@@ -834,7 +850,7 @@ struct BDF {
     return hub;
   }
 
-  __device__ __forceinline__ double ydd_norm(double hg) {
+  __device__ double ydd_norm(double hg) {
     const double yy = zn[0] + hg * zn[1];
     if constexpr (SENS) pb.sens_ydd_prepare(hg);
     double fv = pb.rhs(tn + hg, yy);
@@ -856,7 +872,7 @@ struct BDF {
     return r;
   }
 
-  __device__ __forceinline__ int hin(double tout) {
+  __device__ int hin(double tout) {
     const double tdiff = tout - tn;
     if (tdiff == 0.0) return AMX_TOO_CLOSE;
     const double sign = (tdiff > 0.0) ? 1.0 : -1.0;
@@ -903,7 +919,7 @@ struct BDF {
   // Advances to tout; on success tn == tout and zn[0] holds y(tout).
   // pb.on_step() is invoked for the initial point and after every accepted
   // step (trajectory storage); returning false stops with AMX_TRAJ_FULL.
-  __device__ __forceinline__ int advance(double tout) {
+  __device__ int advance(double tout) {
     tstop = tout;
     tstopset = true;
     if (nst == 0) {
@@ -951,6 +967,7 @@ struct BDF {
         if constexpr (SENS) pb.sens_set_ewt();
       }
       if (nst >= sp.maxsteps) return AMX_TOO_MUCH_WORK;
+      if (sp.deadline != 0 && wall_clock64() > sp.deadline) return AMX_MAX_TIME_EXCEEDED;
       double nrm = wrms(zn[0], ewt);
       if constexpr (QUAD) if (quad_errcon()) nrm = fmax(nrm, wrmsQ(znQ[0], ewtQ));
       if constexpr (SENS) nrm = fmax(nrm, pb.sens_norm_zn(0));

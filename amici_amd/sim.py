@@ -160,6 +160,7 @@ class Solver:
     sensi: SensitivityOrder = SensitivityOrder.none
     sensi_meth: SensitivityMethod = SensitivityMethod.forward
     max_traj: int = 0
+    maxtime: float = 0.0
 
     def set_relative_tolerance(self, v): self.rtol = float(v)
     def get_relative_tolerance(self): return self.rtol
@@ -174,6 +175,8 @@ class Solver:
     def set_max_steps(self, v): self.maxsteps = int(v)
     def get_max_steps(self): return self.maxsteps
     def set_max_steps_backward_problem(self, v): self.maxstepsB = int(v)
+    def set_max_time(self, v): self.maxtime = float(v)
+    def get_max_time(self): return self.maxtime
     def set_sensitivity_order(self, v): self.sensi = SensitivityOrder(int(v))
     def get_sensitivity_order(self): return self.sensi
     def set_sensitivity_method(self, v): self.sensi_meth = SensitivityMethod(int(v))
@@ -193,7 +196,8 @@ class Solver:
                               maxsteps=self.maxsteps, rtol_fsa=self.rtol_fsa,
                               atol_fsa=self.atol_fsa, rtolB=self.rtolB, atolB=self.atolB,
                               quad_rtol=self.quad_rtol, quad_atol=self.quad_atol,
-                              maxstepsB=self.maxstepsB, max_traj=self.max_traj)
+                              maxstepsB=self.maxstepsB, max_traj=self.max_traj,
+                              maxtime=self.maxtime)
 
 
 class ExpData:

@@ -43,6 +43,8 @@ typedef struct {
     int maxstepsB;             /* backward step limit per interval; 0 -> 100 * maxsteps */
     int sensi_meth;            /* AMX_SENSI_* */
     int max_traj;              /* adjoint: stored forward steps per instance (0 -> maxsteps + 1) */
+    double maxtime;            /* wall-clock seconds per instance and problem (0 = unlimited),
+                                  AMICI_MAX_TIME_EXCEEDED when exceeded (Solver::setMaxTime) */
 } amx_solver_opts;
 
 typedef struct {
@@ -83,6 +85,15 @@ int amx_run_batch(const amx_solver_opts* opts, const amx_batch_in* in, amx_batch
  * (sequential launches on `stream`), forward sensitivities or none. */
 int amx_run_sequential(const amx_solver_opts* opts, const amx_batch_in* in, amx_batch_out* out,
                        void* workspace, size_t workspace_bytes, void* stream);
+
+/* Diagnostics: record every accepted step and every failed attempt of the
+ * forward integration of batch instance `instance` into `buffer` (device
+ * memory, `capacity` records of 8 doubles: event (1 step, 2 nonlinear-solver
+ * failure, 3 error-test failure), t, h, order, error norm or flag, cumulative
+ * Newton iterations, linear-solver setups, Jacobian evaluations).  Applies to
+ * the launches that follow; buffer NULL switches tracing off.  Process-wide
+ * setting, not thread-safe. */
+int amx_set_trace(double* buffer, int instance, int capacity);
 
 /* Text for an AMICI status code. */
 const char* amx_status_string(int status);
