@@ -18,7 +18,9 @@ Generators (all seeded with ``numpy.random.default_rng``):
 """
 from __future__ import annotations
 
+import copy
 import dataclasses
+import functools
 import math
 from typing import Sequence
 
@@ -151,6 +153,8 @@ def synthetic_network(
     x0_base: float = 1.0,
     noise: int = NOISE_LOGNORMAL,
     name: str | None = None,
+    t_end: float = 100.0,
+    min_observed: float = 1e-3,
 ) -> ReactionNetwork:
     """Seeded random mass-action network.
 
@@ -161,7 +165,11 @@ def synthetic_network(
     non-negative.  Rate constants are log-uniform in ``10**log10_k_range``.
     ``n_inputs`` distinct species take their initial concentration from a
     fixed parameter; the others start at ``x0_base``.  Observables are sums of
-    ``terms_per_observable`` distinct species with weights U(0.5, 2).
+    ``terms_per_observable`` distinct species with weights U(0.5, 2), drawn
+    from the species whose concentration stays >= ``min_observed`` on
+    [0, t_end] at the nominal rates for inputs 0.1, 1 and 10 (determined once
+    with scipy's stiff integrator), so that log-normal observables stay away
+    from the absolute-tolerance floor.
     """
     rng = np.random.default_rng(seed)
     n_uni = int(round(frac_unimolecular * n_reactions))
@@ -186,9 +194,18 @@ def synthetic_network(
     for j, s in enumerate(inputs):
         x0k[s] = j
         x0c[s] = 0.0
+    probe = ReactionNetwork(
+        name="probe", species=[f"X{i}" for i in range(n_species)], reactions=reactions,
+        parameter_names=[f"k{r}" for r in range(n_reactions)], theta_nominal=theta,
+        fixed_parameter_names=[f"x0_X{int(s)}" for s in inputs], k_nominal=np.ones(n_inputs),
+        x0_const=x0c, x0_fixed=x0k, observable_matrix=np.zeros((0, n_species)), noise=noise)
+    floor = _species_floor(probe, t_end)
+    cand = np.where(floor >= min_observed)[0]
+    if len(cand) < terms_per_observable + 2:
+        cand = np.argsort(floor)[::-1][: max(terms_per_observable + 2, len(cand))]
     c = np.zeros((n_observables, n_species))
     for k in range(n_observables):
-        sp = rng.choice(n_species, size=terms_per_observable, replace=False)
+        sp = rng.choice(cand, size=terms_per_observable, replace=False)
         c[k, sp] = rng.uniform(0.5, 2.0, size=terms_per_observable)
     net = ReactionNetwork(
         name=name or f"massaction_n{n_species}_r{n_reactions}_s{seed}",
@@ -207,16 +224,58 @@ def synthetic_network(
     return net
 
 
-def network_n40() -> ReactionNetwork:
-    """40 species, 120 reactions, k log-uniform 1e-3..1e4, 8 inputs, 30 observables (seed 3)."""
+def _species_floor(net: ReactionNetwork, t_end: float) -> np.ndarray:
+    """Minimum concentration of every species over [0, t_end] at the nominal
+    rates, over input levels 0.1, 1 and 10 (scipy BDF, rtol 1e-6)."""
+    from scipy.integrate import solve_ivp
+
+    S = net.stoichiometry()
+    p = 10.0 ** np.asarray(net.theta_nominal)
+    react = [list(rx.reactants) for rx in net.reactions]
+    kidx = np.array([rx.rate for rx in net.reactions])
+
+    def f(_t, x):
+        w = p[kidx].copy()
+        for r, re in enumerate(react):
+            for a in re:
+                w[r] *= x[a]
+        return S @ w
+
+    def jac(_t, x):
+        J = np.zeros((net.nx, net.nx))
+        for r, re in enumerate(react):
+            for i, a in enumerate(re):
+                d = p[kidx[r]]
+                for j, b in enumerate(re):
+                    if j != i:
+                        d *= x[b]
+                J[:, a] += S[:, r] * d
+        return J
+
+    floor = np.full(net.nx, np.inf)
+    for level in (0.1, 1.0, 10.0):
+        x0 = np.where(net.x0_fixed >= 0, level, net.x0_const)
+        sol = solve_ivp(f, (0.0, t_end), x0, method="BDF", jac=jac, rtol=1e-6, atol=1e-12,
+                        t_eval=np.linspace(0.0, t_end, 201))
+        floor = np.minimum(floor, sol.y.min(axis=1))
+    return floor
+
+
+@functools.lru_cache(maxsize=None)
+def _n40_cached() -> ReactionNetwork:
     return synthetic_network(40, 120, seed=3, log10_k_range=(-3.0, 4.0), n_inputs=8,
                              n_observables=30, name="n40")
+
+
+def network_n40() -> ReactionNetwork:
+    """40 species, 120 reactions, k log-uniform 1e-3..1e4, 8 inputs, 30 observables (seed 3)."""
+    return copy.deepcopy(_n40_cached())
 
 
 def network_n200() -> ReactionNetwork:
     """200 species, 600 reactions, k log-uniform 1e-4..1e5, 20 inputs, 60 observables (seed 4)."""
     return synthetic_network(200, 600, seed=4, log10_k_range=(-4.0, 5.0), n_inputs=20,
-                             n_observables=60, name="n200")
+                             n_observables=60, name="n200", t_end=500.0)
 
 
 def linear_chain(n: int = 3, seed: int = 11) -> ReactionNetwork:

@@ -1,0 +1,217 @@
+"""Pins the oracle (reference CVODES + restated AMICI layer) before it is
+trusted as the checker.
+
+Each restated piece is checked against something the mathematics fixes:
+closed forms (linear ODE and its parameter derivatives via the block matrix
+exponential), invariants (mass conservation, replicate identity), published
+reference values (Robertson), library routines (scipy Radau, numpy), sympy
+derivatives and central finite differences of the log-likelihood (the recipe of
+python/sdist/amici/sim/sundials/gradient_check.py:106-129, epsilon on the log10
+scale).
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy.integrate import solve_ivp
+from scipy.linalg import expm
+
+from amici_amd import codegen, networks
+from oracle import cvodes_oracle as O
+
+
+def _linear_A(net, theta):
+    """xdot = A x for a first-order network."""
+    k = 10.0 ** np.asarray(theta)
+    A = np.zeros((net.nx, net.nx))
+    for rx in net.reactions:
+        (a,), (b,) = rx.reactants, rx.products
+        A[a, a] -= k[rx.rate]
+        A[b, a] += k[rx.rate]
+    return A, k
+
+
+def test_xdot_jac_dfdp_match_sympy_and_fd():
+    rng = np.random.default_rng(0)
+    for net in (networks.robertson(), networks.network_n40(), networks.linear_chain(4)):
+        om = O.OracleModel(net)
+        sm = codegen.symbolic_model(net)
+        theta = net.theta_nominal
+        p = 10.0 ** theta
+        x = rng.uniform(0.1, 2.0, net.nx)
+        subs = {**dict(zip(sm["x"], x)), **dict(zip(sm["p"], p))}
+        f_sym = np.array([float(e.subs(subs)) for e in sm["xdot"]])
+        np.testing.assert_allclose(om.xdot(theta, x), f_sym, rtol=1e-12, atol=1e-12 * np.abs(f_sym).max())
+        J = om.jac(theta, x)
+        J_sym = np.array(sm["J"].subs(subs).evalf(), dtype=float)
+        np.testing.assert_allclose(J, J_sym, rtol=1e-12, atol=1e-12 * np.abs(J_sym).max())
+        # dxdot/dp against sympy and against a finite difference in p
+        dfdp = om.dxdotdp(theta, x)
+        dfdp_sym = np.array(sm["dfdp"].subs(subs).evalf(), dtype=float)
+        np.testing.assert_allclose(dfdp, dfdp_sym, rtol=1e-12, atol=1e-12 * np.abs(dfdp_sym).max())
+        # xdot is linear in each rate constant, so a step of p_j itself is an
+        # exact difference quotient up to roundoff
+        scale = np.abs(om.xdot(theta, x)).max() + np.abs(J @ x).max()
+        for j in range(net.np):
+            tp = theta.copy()
+            tp[j] = np.log10(2.0 * p[j])
+            fd = (om.xdot(tp, x) - om.xdot(theta, x)) / p[j]
+            np.testing.assert_allclose(dfdp[:, j], fd, rtol=1e-9, atol=1e-12 * scale / p[j])
+
+
+def test_noise_closed_forms():
+    s, m = 0.1, 1.7
+    for y in (0.3, 1.7, 2.9):
+        jn = O.Jy(0, y, s, m)
+        assert jn == pytest.approx(0.5 * math.log(2 * math.pi * s * s) + 0.5 * ((y - m) / s) ** 2, rel=1e-14)
+        jl = O.Jy(1, y, s, m)
+        assert jl == pytest.approx(0.5 * math.log(2 * math.pi * s * s * m * m)
+                                   + 0.5 * ((math.log(y) - math.log(m)) / s) ** 2, rel=1e-14)
+        for noise in (0, 1):
+            h = 1e-6
+            fd = (O.Jy(noise, y + h, s, m) - O.Jy(noise, y - h, s, m)) / (2 * h)
+            assert O.dJydy(noise, y, s, m) == pytest.approx(fd, rel=1e-7, abs=1e-9)
+
+
+def test_linear_states_and_fsa_closed_form():
+    net = networks.linear_chain(3)
+    ts = np.array([0.0, 0.1, 0.5, 1.0, 3.0, 10.0])
+    r = O.simulate(net, net.theta_nominal[None], np.zeros((1, 0)), ts, rtol=1e-12, atol=1e-16,
+                   sensi=1, want_sx=True)
+    assert r["status"][0] == 0
+    A, k = _linear_A(net, net.theta_nominal)
+    x0 = net.x0_const
+    n = net.nx
+    for it, t in enumerate(ts):
+        np.testing.assert_allclose(r["x"][0, it], expm(A * t) @ x0, rtol=1e-9, atol=1e-13)
+        for j in range(net.np):
+            dA = np.zeros((n, n))
+            rx = net.reactions[j]
+            (a,), (b,) = rx.reactants, rx.products
+            dA[a, a] -= 1.0
+            dA[b, a] += 1.0
+            big = np.block([[A, dA], [np.zeros((n, n)), A]])
+            dx = (expm(big * t)[:n, n:] @ x0) * k[j] * math.log(10.0)  # log10 scale
+            np.testing.assert_allclose(r["sx"][0, it, j], dx, rtol=1e-7, atol=1e-11)
+
+
+def _fd_sllh(net, theta, kfix, ts, meas, sigma, eps=1e-4, **kw):
+    g = np.zeros(net.np)
+    for j in range(net.np):
+        tp = theta.copy()
+        tp[j] += eps / 2
+        tm = theta.copy()
+        tm[j] -= eps / 2
+        lp = O.simulate(net, tp[None], kfix[None], ts, meas[None], sigma[None], **kw)["llh"][0]
+        lm = O.simulate(net, tm[None], kfix[None], ts, meas[None], sigma[None], **kw)["llh"][0]
+        g[j] = (lp - lm) / eps
+    return g
+
+
+@pytest.mark.parametrize("name", ["linear", "robertson"])
+def test_fsa_asa_fd_agree(name):
+    if name == "linear":
+        net = networks.linear_chain(3)
+        ts = np.linspace(0.0, 5.0, 11)
+    else:
+        net = networks.robertson()
+        ts = networks.robertson_times()[::4]
+    theta = net.theta_nominal.copy()
+    kfix = np.zeros(0)
+    y = O.simulate(net, theta[None], kfix[None], ts, rtol=1e-12, atol=1e-16)["y"][0]
+    sigma = np.full_like(y, 0.01)
+    meas = networks.add_noise(y, net.noise, 0.01, seed=7)
+    kw = dict(rtol=1e-12, atol=1e-16)
+    f = O.simulate(net, theta[None], kfix[None], ts, meas[None], sigma[None], sensi=1, **kw)
+    a = O.simulate(net, theta[None], kfix[None], ts, meas[None], sigma[None], sensi=2,
+                   quad_rtol=1e-12, quad_atol=1e-16, **kw)
+    assert f["status"][0] == 0 and a["status"][0] == 0
+    assert f["llh"][0] == pytest.approx(a["llh"][0], rel=1e-9)
+    scale = np.abs(f["sllh"][0]).max()
+    np.testing.assert_allclose(a["sllh"][0], f["sllh"][0], rtol=1e-6, atol=1e-7 * scale)
+    fd = _fd_sllh(net, theta, kfix, ts, meas, sigma, **kw)
+    np.testing.assert_allclose(f["sllh"][0], fd, rtol=1e-4, atol=1e-5 * scale)
+
+
+def test_replicate_identity():
+    """llh(2 copies of the data) == 2 llh, for FSA and ASA
+    (tests/cpp/jakstat_adjoint/tests1.cpp:73-130)."""
+    net = networks.robertson()
+    ts = networks.robertson_times()[::5]
+    theta = net.theta_nominal + 0.1
+    y = O.simulate(net, net.theta_nominal[None], np.zeros((1, 0)), ts, rtol=1e-10, atol=1e-14)["y"][0]
+    meas = networks.add_noise(y, net.noise, 0.01, seed=3)
+    sig = np.full_like(meas, 0.01)
+    ts2 = np.repeat(ts, 2)
+    meas2 = np.repeat(meas, 2, axis=0)
+    sig2 = np.repeat(sig, 2, axis=0)
+    for sensi in (1, 2):
+        one = O.simulate(net, theta[None], np.zeros((1, 0)), ts, meas[None], sig[None], sensi=sensi)
+        two = O.simulate(net, theta[None], np.zeros((1, 0)), ts2, meas2[None], sig2[None], sensi=sensi)
+        assert two["llh"][0] == pytest.approx(2 * one["llh"][0], rel=1e-9)
+        np.testing.assert_allclose(two["sllh"][0], 2 * one["sllh"][0], rtol=1e-6,
+                                   atol=1e-8 * np.abs(one["sllh"][0]).max())
+
+
+def test_robertson_invariants_and_published_values():
+    net = networks.robertson()
+    ts = np.array([0.4, 4.0, 40.0, 400.0, 4e4])
+    r = O.simulate(net, net.theta_nominal[None], np.zeros((1, 0)), ts, rtol=1e-12, atol=1e-16)
+    x = r["x"][0]
+    np.testing.assert_allclose(x.sum(1), 1.0, atol=1e-12)
+    # reference solution of the Robertson problem at t = 40
+    # (Hairer & Wanner, Solving ODEs II, sect. IV.10; SUNDIALS cvRoberts example)
+    np.testing.assert_allclose(x[2], [0.7158270687, 9.185534765e-6, 0.2841637457], rtol=1e-8)
+    # scipy Radau at tight tolerance on the benchmark time grid
+    tg = networks.robertson_times()
+    rr = O.simulate(net, net.theta_nominal[None], np.zeros((1, 0)), tg, rtol=1e-12, atol=1e-16)
+    om = O.OracleModel(net)
+    sol = solve_ivp(lambda t, y: om.xdot(net.theta_nominal, y), (0, tg[-1]), net.x0_const,
+                    method="Radau", t_eval=tg, rtol=1e-12, atol=1e-18,
+                    jac=lambda t, y: om.jac(net.theta_nominal, y))
+    np.testing.assert_allclose(rr["x"][0], sol.y.T, rtol=1e-6, atol=1e-14)
+
+
+def test_network_against_scipy_radau():
+    net = networks.network_n40()
+    cs = networks.condition_set("N40")
+    om = O.OracleModel(net)
+    for b in (0, 7):
+        x0 = np.where(net.x0_fixed >= 0, cs["kfix"][b][np.maximum(net.x0_fixed, 0)], net.x0_const)
+        r = O.simulate(net, cs["theta"][b][None], cs["kfix"][b][None], cs["ts"], rtol=1e-11,
+                       atol=1e-15)
+        th = cs["theta"][b]
+        sol = solve_ivp(lambda t, y: om.xdot(th, y), (0, cs["ts"][-1]), x0, method="Radau",
+                        t_eval=cs["ts"], rtol=1e-11, atol=1e-15, jac=lambda t, y: om.jac(th, y))
+        np.testing.assert_allclose(r["x"][0], sol.y.T, rtol=1e-6, atol=1e-10)
+
+
+def test_chain_rule_log10():
+    """sllh on the log10 scale equals p ln(10) times the gradient in p
+    (src/rdata.cpp:782-830), by a finite difference in p itself."""
+    net = networks.linear_chain(3)
+    ts = np.linspace(0.0, 4.0, 9)
+    theta = net.theta_nominal
+    y = O.simulate(net, theta[None], np.zeros((1, 0)), ts, rtol=1e-12, atol=1e-16)["y"][0]
+    meas = y + 0.05
+    sig = np.full_like(meas, 0.1)
+    f = O.simulate(net, theta[None], np.zeros((1, 0)), ts, meas[None], sig[None], sensi=1,
+                   rtol=1e-12, atol=1e-16)
+    p = 10.0 ** theta
+    for j in range(net.np):
+        h = 1e-6 * p[j]
+        tp, tm = theta.copy(), theta.copy()
+        tp[j], tm[j] = np.log10(p[j] + h), np.log10(p[j] - h)
+        lp = O.simulate(net, tp[None], np.zeros((1, 0)), ts, meas[None], sig[None], rtol=1e-12, atol=1e-16)["llh"][0]
+        lm = O.simulate(net, tm[None], np.zeros((1, 0)), ts, meas[None], sig[None], rtol=1e-12, atol=1e-16)["llh"][0]
+        assert f["sllh"][0][j] == pytest.approx((lp - lm) / (2 * h) * p[j] * math.log(10.0), rel=1e-5)
+
+
+def test_batch_threads_deterministic():
+    """run_simulations(num_threads>1) gives the same results as one thread
+    (python/tests/test_pregenerated_models.py:148-174)."""
+    net = networks.robertson()
+    th = networks.robertson_parameter_sets(16, 2)
+    a = O.simulate(net, th, np.zeros((16, 0)), networks.robertson_times(), nthreads=1, rtol=1e-8, atol=1e-12)
+    b = O.simulate(net, th, np.zeros((16, 0)), networks.robertson_times(), nthreads=4, rtol=1e-8, atol=1e-12)
+    np.testing.assert_array_equal(a["x"], b["x"])
