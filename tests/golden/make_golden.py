@@ -1,0 +1,122 @@
+"""Writes the golden fixtures under tests/golden/ from the CVODES oracle.
+
+The reference's own golden results (tests/cpp/expected_results_py.h5) are a
+Git LFS pointer in the reference checkout, so the fixtures are produced by the
+oracle: SUNDIALS CVODES 7.1.1 from the reference sources, driven by the C
+restatement of AMICI's forward/backward problem (oracle/cvodes_harness.c), at
+rtol 1e-10 / atol 1e-14 for values and at rtol 1e-8 / atol 1e-12 for the
+solver counters that define the algorithmic work of one evaluation.
+
+Each fixture holds the inputs (theta, kfix, ts, meas, sigma) together with
+the expected outputs, so the GPU parity tests need neither the reference nor
+the oracle library at run time.
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+
+from amici_amd import networks  # noqa: E402
+from oracle import cvodes_oracle as O  # noqa: E402
+
+OUT = Path(__file__).resolve().parent
+TIGHT = dict(rtol=1e-10, atol=1e-14, quad_rtol=1e-10, quad_atol=1e-14)
+BENCH = dict(rtol=1e-8, atol=1e-12, quad_rtol=1e-8, quad_atol=1e-12)
+
+
+def measurements(name: str, nb: int | None = None):
+    """Synthetic data of a configuration: the oracle's observables at the
+    nominal parameters (per-condition inputs kept), plus the configuration's
+    noise with its seed."""
+    cs = networks.condition_set(name)
+    net = cs["network"]
+    theta = cs["theta"][:nb]
+    kfix = cs["kfix"][:nb]
+    nom = np.repeat(net.theta_nominal[None, :], theta.shape[0], 0)
+    y = O.simulate(net, nom, kfix, cs["ts"], nthreads=8, **TIGHT)["y"]
+    meas = networks.add_noise(y, net.noise, cs["sigma"], seed=cs["noise_seed"])
+    sigma = np.full_like(meas, cs["sigma"])
+    return cs, theta, kfix, meas, sigma
+
+
+def run(net, theta, kfix, ts, meas, sigma, sensi, opts, **kw):
+    r = O.simulate(net, theta, kfix, ts, meas, sigma, sensi=sensi, nthreads=8, **opts, **kw)
+    assert (r["status"] == 0).all(), r["status"]
+    return r
+
+
+def main() -> None:
+    # R: one nominal Robertson instance, states + FSA + ASA
+    cs, theta, kfix, meas, sigma = measurements("R")
+    net, ts = cs["network"], cs["ts"]
+    f = run(net, theta, kfix, ts, meas, sigma, 1, TIGHT, want_sx=True)
+    a = run(net, theta, kfix, ts, meas, sigma, 2, TIGHT)
+    c0 = run(net, theta, kfix, ts, meas, sigma, 0, BENCH)["counters"]
+    c1 = run(net, theta, kfix, ts, meas, sigma, 1, BENCH)["counters"]
+    c2 = run(net, theta, kfix, ts, meas, sigma, 2, BENCH)["counters"]
+    np.savez_compressed(OUT / "robertson_nominal.npz", theta=theta, kfix=kfix, ts=ts, meas=meas,
+                        sigma=sigma, x=f["x"], y=f["y"], sx=f["sx"], llh=f["llh"],
+                        sllh_fsa=f["sllh"], sllh_asa=a["sllh"], counters_states=c0,
+                        counters_fsa=c1, counters_asa=c2)
+
+    # RB: the first 64 of the 4096 Robertson parameter sets
+    cs, theta, kfix, meas, sigma = measurements("RB", 64)
+    f = run(net, theta, kfix, ts, meas, sigma, 1, TIGHT)
+    a = run(net, theta, kfix, ts, meas, sigma, 2, TIGHT)
+    np.savez_compressed(OUT / "robertson_batch64.npz", theta=theta, kfix=kfix, ts=ts, meas=meas,
+                        sigma=sigma, x=f["x"], y=f["y"], llh=f["llh"], sllh_fsa=f["sllh"],
+                        sllh_asa=a["sllh"])
+
+    # N40: 16 of the 1024 input conditions
+    cs, theta, kfix, meas, sigma = measurements("N40", 16)
+    net, ts = cs["network"], cs["ts"]
+    f = run(net, theta, kfix, ts, meas, sigma, 1, TIGHT)
+    a = run(net, theta, kfix, ts, meas, sigma, 2, TIGHT)
+    c2 = run(net, theta, kfix, ts, meas, sigma, 2, BENCH)["counters"]
+    c1 = run(net, theta, kfix, ts, meas, sigma, 1, BENCH)["counters"]
+    np.savez_compressed(OUT / "n40_conditions16.npz", theta=theta, kfix=kfix, ts=ts, meas=meas,
+                        sigma=sigma, x=f["x"], y=f["y"], llh=f["llh"], sllh_fsa=f["sllh"],
+                        sllh_asa=a["sllh"], counters_fsa=c1, counters_asa=c2)
+
+    # edge cases on Robertson: missing data (NaN), a repeated output time,
+    # output at t0 only, and a measurement-free instance
+    net = networks.robertson()
+    ts_e = np.array([0.0, 0.0, 1e-3, 0.5, 0.5, 10.0, 1e3])
+    th = networks.robertson_parameter_sets(3, seed=9)
+    y = run(net, th, np.zeros((3, 0)), ts_e, None, None, 0, TIGHT)["y"]
+    meas_e = networks.add_noise(y, net.noise, 0.01, seed=9)
+    meas_e[0, 2, 1] = np.nan
+    meas_e[1, :, 0] = np.nan
+    meas_e[2, :, :] = np.nan
+    sig_e = np.full_like(meas_e, 0.02)
+    f = run(net, th, np.zeros((3, 0)), ts_e, meas_e, sig_e, 1, TIGHT)
+    a = run(net, th, np.zeros((3, 0)), ts_e, meas_e, sig_e, 2, TIGHT)
+    np.savez_compressed(OUT / "robertson_edge.npz", theta=th, kfix=np.zeros((3, 0)), ts=ts_e,
+                        meas=meas_e, sigma=sig_e, x=f["x"], y=f["y"], llh=f["llh"],
+                        sllh_fsa=f["sllh"], sllh_asa=a["sllh"])
+
+    # linear chain: closed-form-pinned small model for FSA / ASA cross-checks
+    net = networks.linear_chain(3)
+    ts_l = np.linspace(0.0, 5.0, 11)
+    th = np.stack([net.theta_nominal, net.theta_nominal + 0.3])
+    y = run(net, th, np.zeros((2, 0)), ts_l, None, None, 0, TIGHT)["y"]
+    meas_l = networks.add_noise(y, net.noise, 0.05, seed=5)
+    sig_l = np.full_like(meas_l, 0.05)
+    f = run(net, th, np.zeros((2, 0)), ts_l, meas_l, sig_l, 1, TIGHT, want_sx=True)
+    a = run(net, th, np.zeros((2, 0)), ts_l, meas_l, sig_l, 2, TIGHT)
+    np.savez_compressed(OUT / "linear_chain3.npz", theta=th, kfix=np.zeros((2, 0)), ts=ts_l,
+                        meas=meas_l, sigma=sig_l, x=f["x"], y=f["y"], sx=f["sx"], llh=f["llh"],
+                        sllh_fsa=f["sllh"], sllh_asa=a["sllh"])
+    for p in sorted(OUT.glob("*.npz")):
+        print(p.name, p.stat().st_size)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,99 @@
+"""GPU parity: the HIP batch path (through the C ABI) against the CVODES
+oracle's golden fixtures (tests/golden/make_golden.py, rtol 1e-10).
+
+Tolerances (stated per quantity, from the required accuracy of the path):
+states and observables within 1e-6 relative, sllh within 1e-5 relative, each
+with an absolute floor given below for entries that are zero in exact
+arithmetic (the floor is a fraction of the largest magnitude of the quantity
+in that instance).  The GPU runs at the oracle's tolerances (rtol 1e-10,
+atol 1e-14) so that differences are the two solvers' own truncation errors.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from amici_amd import networks
+
+GOLD = __import__("pathlib").Path(__file__).resolve().parent / "golden"
+TIGHT = dict(rtol=1e-10, atol=1e-14, quad_rtol=1e-10, quad_atol=1e-14, maxsteps=200000,
+             maxstepsB=400000, maxtime=60.0)
+
+pytestmark = pytest.mark.gpu
+
+
+def _load(name):
+    return dict(np.load(GOLD / f"{name}.npz"))
+
+
+def _close(a, b, rtol, floor_frac, what):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    floor = floor_frac * np.max(np.abs(b).reshape(b.shape[0], -1), axis=1)
+    floor = floor.reshape((-1,) + (1,) * (b.ndim - 1))
+    err = np.abs(a - b) / (rtol * np.abs(b) + floor)
+    worst = np.unravel_index(np.argmax(err), err.shape)
+    assert err.max() <= 1.0, (f"{what}: worst at {worst}: got {a[worst]!r} want {b[worst]!r} "
+                              f"(ratio {err.max():.3g})")
+
+
+@pytest.fixture(scope="module")
+def models(gpu):
+    from amici_amd import sim
+    return {"robertson": sim.Model(networks.robertson()),
+            "n40": sim.Model(networks.network_n40()),
+            "linear3": sim.Model(networks.linear_chain(3))}
+
+
+def _run(models, model, g, meth, **over):
+    import torch
+    from amici_amd import sim
+    sol = sim.Solver(**{**TIGHT, **over})
+    if meth:
+        sol.set_sensitivity_order(sim.SensitivityOrder.first)
+        sol.set_sensitivity_method(meth)
+    out = sim.simulate_batch(models[model], sol, g["theta"], g["kfix"], g["ts"], g["meas"],
+                             g["sigma"], want_sx=(meth == sim.SensitivityMethod.forward))
+    torch.cuda.synchronize()
+    return {k: (v.cpu().numpy() if hasattr(v, "cpu") else v) for k, v in out.items()
+            if k != "_keepalive"}
+
+
+CASES = [("robertson", "robertson_nominal"), ("robertson", "robertson_batch64"),
+         ("robertson", "robertson_edge"), ("linear3", "linear_chain3"),
+         ("n40", "n40_conditions16")]
+
+
+@pytest.mark.parametrize("model,fixture", CASES)
+def test_states_observables(models, model, fixture):
+    g = _load(fixture)
+    out = _run(models, model, g, 0)
+    assert (out["status"] == 0).all(), out["status"]
+    _close(out["x"], g["x"], 1e-6, 1e-9, "x")
+    _close(out["y"], g["y"], 1e-6, 1e-9, "y")
+    _close(out["llh"][:, None], g["llh"][:, None], 1e-8, 0.0, "llh")
+
+
+@pytest.mark.parametrize("model,fixture", CASES)
+def test_forward_sensitivities(models, model, fixture):
+    from amici_amd import sim
+    g = _load(fixture)
+    out = _run(models, model, g, sim.SensitivityMethod.forward)
+    assert (out["status"] == 0).all(), out["status"]
+    _close(out["x"], g["x"], 1e-6, 1e-9, "x")
+    _close(out["llh"][:, None], g["llh"][:, None], 1e-8, 0.0, "llh")
+    _close(out["sllh"], g["sllh_fsa"], 1e-5, 1e-7, "sllh (forward)")
+    if "sx" in g:
+        _close(out["sx"], g["sx"], 1e-5, 1e-7, "sx")
+
+
+@pytest.mark.parametrize("model,fixture", CASES)
+def test_adjoint_gradient(models, model, fixture):
+    from amici_amd import sim
+    g = _load(fixture)
+    out = _run(models, model, g, sim.SensitivityMethod.adjoint)
+    assert (out["status"] == 0).all(), out["status"]
+    _close(out["llh"][:, None], g["llh"][:, None], 1e-8, 0.0, "llh")
+    _close(out["sllh"], g["sllh_asa"], 1e-5, 1e-7, "sllh (adjoint)")
+    # the oracle's own forward and adjoint gradients agree to the same bar
+    _close(g["sllh_asa"], g["sllh_fsa"], 1e-5, 1e-7, "oracle sllh asa vs fsa")
