@@ -14,7 +14,7 @@ AMX_NSTATS = 16
 SENSI_NONE, SENSI_FORWARD, SENSI_ADJOINT = 0, 1, 2
 
 SYMBOLS = ("amx_model_info", "amx_workspace_bytes", "amx_run_batch", "amx_run_sequential",
-           "amx_set_trace", "amx_status_string")
+           "amx_set_trace", "amx_set_timing", "amx_kernel_times", "amx_status_string")
 
 
 class SolverOpts(ctypes.Structure):
@@ -71,6 +71,10 @@ class ModelLib:
             fn.restype = ctypes.c_int
         L.amx_set_trace.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         L.amx_set_trace.restype = ctypes.c_int
+        L.amx_set_timing.argtypes = [ctypes.c_int]
+        L.amx_set_timing.restype = ctypes.c_int
+        L.amx_kernel_times.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
+        L.amx_kernel_times.restype = ctypes.c_int
         L.amx_status_string.argtypes = [ctypes.c_int]
         L.amx_status_string.restype = ctypes.c_char_p
         dims = (ctypes.c_int * 8)()
@@ -82,6 +86,20 @@ class ModelLib:
 
     def status_string(self, s: int) -> str:
         return self.lib.amx_status_string(int(s)).decode()
+
+    def set_timing(self, enable: bool) -> None:
+        self.lib.amx_set_timing(int(bool(enable)))
+
+    def kernel_times(self) -> dict:
+        """Summed forward / backward kernel milliseconds and launch counts since
+        the previous read (HIP events on the launch stream)."""
+        ms = (ctypes.c_double * 2)()
+        n = (ctypes.c_int * 2)()
+        dropped = self.lib.amx_kernel_times(ms, n)
+        if dropped < 0:
+            raise RuntimeError(f"amx_kernel_times failed ({dropped})")
+        return dict(fwd_ms=ms[0], bwd_ms=ms[1], fwd_launches=n[0], bwd_launches=n[1],
+                    dropped=dropped)
 
     def workspace_bytes(self, opts: SolverOpts, nb: int, nt: int) -> int:
         return int(self.lib.amx_workspace_bytes(ctypes.byref(opts), int(nb), int(nt)))

@@ -1,0 +1,236 @@
+"""Benchmark: adjoint NLL+gradient evaluations per second of the 40-species /
+120-reaction mass-action network over 1024 input conditions (rtol 1e-8,
+atol 1e-12, FP64) on one MI355X, beside the plain GPU BDF (one instance per
+launch, forward sensitivities) and the CVODES CPU path.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+One step = one batched adjoint NLL+gradient evaluation of all 1024
+conditions (llh and the 120-entry sllh of each), inputs resident in HBM.
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "NLL+gradient evals/s at rtol 1e-8: batched adjoint vs plain GPU BDF, 1 GPU"
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+C_JY = 16  # flops of Jy + dJy/dy per observable and time point (log-normal)
+TOL = dict(rtol=1e-8, atol=1e-12, quad_rtol=1e-8, quad_atol=1e-12, maxsteps=10000)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def flop_model(counters: np.ndarray, n: int, ops: dict, nt: int, ny: int) -> tuple:
+    """Algorithmic FLOPs per instance of the forward and of the backward kernel,
+    from CVODES counters at rtol 1e-8 (dense-equivalent LU and solve costs):
+
+      forward  = nfe c_f + nje c_J + nsetups (2/3 n^3 + n^2) + nni 2 n^2
+                 + nt (c_y + ny C_JY + 2 ny n)
+      backward = nfeB (2 n^2 + c_fpT) + nsetupsB (2/3 n^3 + n^2) + nniB 2 n^2
+                 + nfQB c_fpT,   c_fpT = 2 nnz(df/dp)
+    """
+    c = counters.astype(np.float64)
+    lu = 2.0 / 3.0 * n ** 3 + n ** 2
+    sv = 2.0 * n * n
+    c_fpT = 2.0 * ops["c_fp"]
+    fwd = (c[:, 1] * ops["c_f"] + c[:, 3] * ops["c_J"] + c[:, 2] * lu + c[:, 4] * sv
+           + nt * (ops["c_y"] + ny * C_JY + 2 * ny * n))
+    bwd = c[:, 10] * (sv + c_fpT) + c[:, 11] * lu + c[:, 12] * sv + c[:, 13] * c_fpT
+    return fwd, bwd
+
+
+def traffic_from_profile():
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 counter summary, if present."""
+    p = ROOT / "profiles" / "n40_adjoint_hbm_traffic.json"
+    if not p.exists():
+        return None
+    try:
+        return json.loads(p.read_text())
+    except ValueError:
+        return None
+
+
+def cpu_baseline(cs, theta, kfix, meas, sigma, sample: int):
+    """The reference's CVODES (oracle/_ref, built from the reference sources)
+    on the host cores: adjoint NLL+gradient of the first `sample` conditions."""
+    try:
+        from oracle import cvodes_oracle as O
+        O.lib()
+    except Exception as e:  # library not built on this machine
+        log(f"cpu baseline skipped: {e}")
+        return None
+    cores = min(16, len(os.sched_getaffinity(0)))
+    t0 = time.perf_counter()
+    r = O.simulate(cs["network"], theta[:sample], kfix[:sample], cs["ts"], meas[:sample],
+                   sigma[:sample], sensi=2, nthreads=cores, **TOL)
+    dt = time.perf_counter() - t0
+    ok = int((r["status"] == 0).sum())
+    return {"value": sample / dt, "unit": "evals/s", "cores": cores, "kind": "reference",
+            "sample": f"{sample} of 1024 N40 conditions, adjoint NLL+gradient, CVODES 7.1.1 "
+                      f"(reference sources) + AMICI layer restated in C, OpenMP over conditions, "
+                      f"{dt:.2f} s wall, {ok}/{sample} succeeded"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--plain-sample", type=int, default=8,
+                    help="conditions timed on the plain GPU BDF path")
+    ap.add_argument("--cpu-sample", type=int, default=128,
+                    help="conditions timed on the CPU baseline (0 = skip)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the batched path shards by instances across GPUs "
+              "with no collective on the hot path, and only the 1-GPU figure is measured.",
+              file=sys.stderr)
+        return 2
+
+    import torch
+
+    from amici_amd import codegen, networks, sim
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a ROCm GPU", file=sys.stderr)
+        return 1
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    cs = networks.condition_set("N40")
+    net = cs["network"]
+    model = sim.Model(net)
+    nb, nt = cs["theta"].shape[0], len(cs["ts"])
+    theta = torch.as_tensor(cs["theta"], device=dev)
+    kfix = torch.as_tensor(cs["kfix"], device=dev)
+    ts = torch.as_tensor(cs["ts"], device=dev)
+
+    # synthetic data: this path's own observables at the nominal parameters,
+    # with multiplicative log-normal noise (sigma 0.1, seed 3)
+    states = sim.Solver(**TOL)
+    out = sim.simulate_batch(model, states, theta, kfix, ts, want_x=False)
+    torch.cuda.synchronize()
+    if int((out["status"] != 0).sum()):
+        raise RuntimeError("data generation failed")
+    y = out["y"].cpu().numpy()
+    meas_np = networks.add_noise(y, net.noise, cs["sigma"], seed=cs["noise_seed"])
+    sigma_np = np.full_like(meas_np, cs["sigma"])
+    meas = torch.as_tensor(meas_np, device=dev)
+    sigma = torch.as_tensor(sigma_np, device=dev)
+
+    adj = sim.Solver(**TOL)
+    adj.set_sensitivity_order(sim.SensitivityOrder.first)
+    adj.set_sensitivity_method(sim.SensitivityMethod.adjoint)
+
+    def step():
+        return sim.simulate_batch(model, adj, theta, kfix, ts, meas, sigma, want_x=False,
+                                  want_y=False, want_stats=(False))
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    model.lib.set_timing(True)
+    model.lib.kernel_times()  # discard warm-up records
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    kt = model.lib.kernel_times()
+    model.lib.set_timing(False)
+    status = res["status"].cpu().numpy()
+    sllh = res["sllh"].cpu().numpy()
+    n_ok = int((status == 0).sum())
+    if n_ok != nb or not np.isfinite(sllh).all():
+        log(f"warning: {nb - n_ok} instances failed: {np.unique(status, return_counts=True)}")
+    value = nb * args.steps / dt
+
+    # solver counters of this path (for the record) and the CVODES counters
+    # that define the algorithmic work
+    stats = sim.simulate_batch(model, adj, theta, kfix, ts, meas, sigma, want_x=False,
+                               want_y=False)["stats"].cpu().numpy()
+    gold = np.load(ROOT / "tests" / "golden" / "n40_counters1024.npz")["counters_asa"]
+    ops = codegen.op_counts(net)
+    f_fwd, f_bwd = flop_model(gold, net.nx, ops, nt, net.ny)
+    fwd_ms = kt["fwd_ms"] / max(kt["fwd_launches"], 1)
+    bwd_ms = kt["bwd_ms"] / max(kt["bwd_launches"], 1)
+    dom = "bwd" if bwd_ms >= fwd_ms else "fwd"
+    dom_flops = float((f_bwd if dom == "bwd" else f_fwd).sum())
+    dom_ms = bwd_ms if dom == "bwd" else fwd_ms
+    achieved = dom_flops / (dom_ms * 1e-3) / 1e12
+    tr = traffic_from_profile()
+    roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
+                "traffic": (tr or {}).get("bytes_per_launch") if tr else None,
+                "kernel": "amx_bwd_adj" if dom == "bwd" else "amx_fwd_adj",
+                "kernel_ms": dom_ms, "flops_per_launch": dom_flops,
+                "fwd_kernel_ms": fwd_ms, "bwd_kernel_ms": bwd_ms,
+                "fwd_flops_per_launch": float(f_fwd.sum()),
+                "bwd_flops_per_launch": float(f_bwd.sum()),
+                "whole_step_tflops": float((f_fwd + f_bwd).sum()) / (dt / args.steps) / 1e12}
+
+    # plain GPU BDF: one instance per launch, forward sensitivities
+    fsa = sim.Solver(**TOL)
+    fsa.set_sensitivity_order(sim.SensitivityOrder.first)
+    fsa.set_sensitivity_method(sim.SensitivityMethod.forward)
+    S = args.plain_sample
+    sl = slice(0, S)
+    sim.simulate_batch(model, fsa, theta[:1], kfix[:1], ts, meas[:1], sigma[:1], sequential=True,
+                       want_x=False, want_y=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pl = sim.simulate_batch(model, fsa, theta[sl], kfix[sl], ts, meas[sl], sigma[sl],
+                            sequential=True, want_x=False, want_y=False)
+    torch.cuda.synchronize()
+    plain_dt = time.perf_counter() - t0
+    plain_ok = int((pl["status"] == 0).sum())
+    plain = {"value": S / plain_dt, "unit": "evals/s",
+             "sample": f"first {S} of {nb} conditions, {plain_ok}/{S} succeeded, one launch each"}
+    # agreement of the two gradients on the sample (adjoint vs forward)
+    a_s = sllh[:S]
+    f_s = pl["sllh"].cpu().numpy()
+    grad_rel = float(np.max(np.abs(a_s - f_s) / (np.abs(f_s) + 1e-7 * np.abs(f_s).max())))
+
+    cpu = cpu_baseline(cs, cs["theta"], cs["kfix"], meas_np, sigma_np, args.cpu_sample) \
+        if args.cpu_sample > 0 else None
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "N40 adjoint NLL+gradient: 40 species, 120 reactions, "
+                               "1024 input conditions, 25 times, 30 log-normal observables",
+                   "instances": nb, "np": net.np, "rtol": TOL["rtol"], "atol": TOL["atol"],
+                   "sensitivity": "adjoint"},
+        "instances_ok": n_ok,
+        "plain_gpu_bdf": plain,
+        "speedup_vs_plain_gpu_bdf": value / plain["value"],
+        "sllh_adjoint_vs_forward_rel": grad_rel,
+        "solver_steps_mean": {"fwd": float(stats[:, 0].mean()), "bwd": float(stats[:, 8].mean()),
+                              "cvodes_fwd": float(gold[:, 0].mean()),
+                              "cvodes_bwd": float(gold[:, 9].mean())},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

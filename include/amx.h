@@ -95,6 +95,19 @@ int amx_run_sequential(const amx_solver_opts* opts, const amx_batch_in* in, amx_
  * setting, not thread-safe. */
 int amx_set_trace(double* buffer, int instance, int capacity);
 
+/* Benchmark instrumentation (no counterpart in the reference): while enabled,
+ * every amx_run_batch / amx_run_sequential launch records HIP events on its
+ * stream around its forward kernel and, for the adjoint, its backward kernel
+ * (up to 4096 launches between reads).  Not for use under graph capture.
+ * Process-wide setting, not thread-safe. */
+int amx_set_timing(int enable);
+
+/* Waits for the recorded launches and returns the summed kernel durations
+ * since the previous call: ms[0] forward kernels, ms[1] backward kernels,
+ * launches[0] / launches[1] how many of each were timed.  Returns the number
+ * of launches that were not recorded (table full), or -30 on a HIP error. */
+int amx_kernel_times(double* ms, int* launches);
+
 /* Text for an AMICI status code. */
 const char* amx_status_string(int status);
 

@@ -1,0 +1,136 @@
+"""The C-ABI model modules load on the CPU and export every entry point that
+include/amx.h declares; host-side argument checks and workspace sizing work
+without touching the GPU."""
+from __future__ import annotations
+
+import ctypes
+import math
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from amici_amd import _lib, codegen, networks
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def _declared():
+    text = (ROOT / "include" / "amx.h").read_text()
+    return sorted(set(re.findall(r"^\s*(?:int|size_t|const char\*)\s+(amx_\w+)\s*\(", text, re.M)))
+
+
+@pytest.fixture(scope="module", params=["robertson", "n40", "linear3"])
+def lib(request):
+    net = {"robertson": networks.robertson, "n40": networks.network_n40,
+           "linear3": lambda: networks.linear_chain(3)}[request.param]()
+    so = codegen.compile_model(net)
+    return net, _lib.ModelLib(so)
+
+
+def test_header_declares_the_binding_symbols():
+    assert set(_declared()) == set(_lib.SYMBOLS)
+
+
+def test_exports_and_dims(lib):
+    net, L = lib
+    for s in _declared():
+        assert hasattr(L.lib, s), s
+    assert (L.nx, L.nr, L.np, L.nk, L.ny) == (net.nx, net.nr, net.np, net.nk, net.ny)
+    assert L.team == 64 and L.noise == int(net.noise)
+    assert L.status_string(-1) == "AMICI_TOO_MUCH_WORK"
+    assert L.status_string(-1001) == "AMICI_NOT_RUN"
+
+
+def test_workspace_bytes(lib):
+    net, L = lib
+    o = _lib.make_opts(sensi_meth=_lib.SENSI_ADJOINT, maxsteps=1000)
+    b1 = L.workspace_bytes(o, 1, 10)
+    b8 = L.workspace_bytes(o, 8, 10)
+    # trajectory: t, y, f per stored step and instance dominate
+    assert b8 - b1 >= 7 * 1001 * (1 + 2 * net.nx) * 8
+    o.max_traj = 100
+    assert L.workspace_bytes(o, 8, 10) < b8
+    f = _lib.make_opts(sensi_meth=_lib.SENSI_FORWARD)
+    assert L.workspace_bytes(f, 4, 10) >= 4 * net.np * net.nx * 8
+    assert L.workspace_bytes(_lib.make_opts(), 4, 10) <= 512
+
+
+def test_invalid_arguments_rejected_on_host(lib):
+    net, L = lib
+    fn = L.lib.amx_run_batch
+    o = _lib.make_opts(sensi_meth=_lib.SENSI_FORWARD)
+    ins = _lib.BatchIn(1, 3, 0, None, None, None, None, None)
+    outs = _lib.BatchOut()
+    assert fn(ctypes.byref(o), ctypes.byref(ins), ctypes.byref(outs), None, 0, None) == -22
+    assert fn(None, ctypes.byref(ins), ctypes.byref(outs), None, 0, None) == -22
+    empty = _lib.BatchIn(0, 3, 0, None, None, None, None, None)
+    assert fn(ctypes.byref(o), ctypes.byref(empty), ctypes.byref(outs), None, 0, None) == 0
+    # a non-null but too-small workspace is refused before any launch
+    buf = (ctypes.c_double * 4)()
+    p = ctypes.cast(buf, ctypes.c_void_p).value
+    ins = _lib.BatchIn(1, 3, 0, p, p if net.nk else None, p, None, p)
+    outs = _lib.BatchOut(None, None, None, p, p, p, None)
+    assert fn(ctypes.byref(o), ctypes.byref(ins), ctypes.byref(outs), p, 8, None) == -23
+    bad = _lib.make_opts(rtol=-1.0)
+    assert fn(ctypes.byref(bad), ctypes.byref(ins), ctypes.byref(outs), p, 8, None) == -22
+    bad = _lib.make_opts(rtol=math.nan)
+    assert fn(ctypes.byref(bad), ctypes.byref(ins), ctypes.byref(outs), p, 8, None) == -22
+    # the sequential path has no adjoint
+    adj = _lib.make_opts(sensi_meth=_lib.SENSI_ADJOINT)
+    assert L.lib.amx_run_sequential(ctypes.byref(adj), ctypes.byref(ins), ctypes.byref(outs),
+                                    p, 1 << 30, None) == -22
+
+
+def test_timing_read_without_launches(lib):
+    _, L = lib
+    L.set_timing(True)
+    t = L.kernel_times()
+    L.set_timing(False)
+    assert t["fwd_launches"] == 0 and t["bwd_launches"] == 0 and t["dropped"] == 0
+
+
+def test_tables_match_sympy():
+    """The constexpr sparsity tables emitted into the model module, read the
+    way the device code reads them, agree with the sympy derivation of xdot,
+    J, J^T v, dxdot/dp and y."""
+    rng = np.random.default_rng(1)
+    for net in (networks.robertson(), networks.network_n40(), networks.linear_chain(3)):
+        t = codegen.model_tables(net)
+        sm = codegen.symbolic_model(net)
+        x = rng.uniform(0.1, 2.0, net.nx)
+        k = 10.0 ** net.theta_nominal
+        v = rng.standard_normal(net.nx)
+        subs = {**dict(zip(sm["x"], x)), **dict(zip(sm["p"], k))}
+        xv = lambda i: 1.0 if i < 0 else x[i]  # noqa: E731
+        w = np.array([k[t["KI"][r]] * xv(t["RA"][r]) * xv(t["RB"][r]) for r in range(net.nr)])
+        f = np.array([sum(t["SP_C"][e] * w[t["SP_R"][e]]
+                          for e in range(t["SP_PTR"][i], t["SP_PTR"][i + 1]))
+                      for i in range(net.nx)])
+        f_sym = np.array([float(e.subs(subs)) for e in sm["xdot"]])
+        np.testing.assert_allclose(f, f_sym, rtol=1e-12, atol=1e-12 * np.abs(f_sym).max())
+        J = np.zeros((net.nx, net.nx))
+        for q in range(t["JNZ"]):
+            for e in range(t["JT_PTR"][q], t["JT_PTR"][q + 1]):
+                J[t["J_I"][q], t["J_J"][q]] += t["JT_C"][e] * k[t["KI"][t["JT_R"][e]]] * xv(t["JT_O"][e])
+        J_sym = np.array(sm["J"].subs(subs).evalf(), dtype=float)
+        np.testing.assert_allclose(J, J_sym, rtol=1e-12, atol=1e-12 * np.abs(J_sym).max())
+        # J^T v through the per-state reaction list (dw_r/dx_s) and S columns
+        jtv = np.zeros(net.nx)
+        for s_ in range(net.nx):
+            for e in range(t["DX_PTR"][s_], t["DX_PTR"][s_ + 1]):
+                r = t["DX_R"][e]
+                sv = sum(t["RS_C"][q] * v[t["RS_S"][q]] for q in range(t["RS_PTR"][r], t["RS_PTR"][r + 1]))
+                jtv[s_] += t["DX_C"][e] * k[t["KI"][r]] * xv(t["DX_O"][e]) * sv
+        np.testing.assert_allclose(jtv, J_sym.T @ v, rtol=1e-10, atol=1e-12 * np.abs(J_sym).max())
+        # dxdot/dp_j = sum over the reactions of rate j of S[:, r] w_r / k_j
+        dfdp_sym = np.array(sm["dfdp"].subs(subs).evalf(), dtype=float)
+        S = net.stoichiometry()
+        for j in range(net.np):
+            col = sum(S[:, r] * w[r] / k[j] for r in t["PR_R"][t["PR_PTR"][j]:t["PR_PTR"][j + 1]])
+            np.testing.assert_allclose(col, dfdp_sym[:, j], rtol=1e-12, atol=1e-14)
+        y = np.array([sum(t["OB_C"][e] * x[t["OB_S"][e]] for e in range(t["OB_PTR"][q], t["OB_PTR"][q + 1]))
+                      for q in range(net.ny)])
+        y_sym = np.array([float(e.subs(subs)) for e in sm["y"]])
+        np.testing.assert_allclose(y, y_sym, rtol=1e-14)
