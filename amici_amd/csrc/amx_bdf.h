@@ -33,6 +33,13 @@ constexpr double ETA_MAX_FX = 1.5, ETA_MAX_FS = 10000.0, ETA_MAX_ES = 10.0,
                  ETA_MIN_EF = 0.1, ETA_CF = 0.25;
 constexpr int SMALL_NST = 10, SMALL_NEF = 2, MXNEF1 = 3, LONG_WAIT = 10,
               MXNCF = 10, MXNEF = 7, NLS_MAXCOR = 3, MSBP = 20, MSBJ = 51;
+// Upper bound on the attempts of one step before the failure counters stop
+// it: the state / quadrature counters (ncf < MXNCF, nef, nefQ < MXNEF) are
+// reset each time the sensitivity stage is entered, and that stage allows
+// ncfS < MXNCF, nefS < MXNEF, so at most (MXNCF + MXNEF - 1) *
+// (MXNCF + 2 MXNEF - 2) attempts occur; the guard sits above that bound and
+// only ends a step that the counters would not end.
+constexpr int MAX_ATTEMPTS = (MXNCF + MXNEF) * (MXNCF + 2 * MXNEF);
 constexpr double ADDON = 1e-6, BIAS1 = 6.0, BIAS2 = 6.0, BIAS3 = 10.0,
                  DGMAX_LSETUP = 0.3, DGMAX_JBAD = 0.2, CRDOWN = 0.3,
                  RDIV = 2.0, CORTES = 0.1, FUZZ_FACTOR = 100.0,
@@ -782,7 +789,7 @@ struct BDF {
     int nflag = FIRST_CALL;
     double dsm = 0.0;
     for (int attempt = 0;; ++attempt) {
-      if (attempt >= MXNCF + MXNEF + 8) return AMX_CONV_FAILURE;
+      if (attempt >= MAX_ATTEMPTS) return AMX_CONV_FAILURE;
       predict();
       set_coeffs();
       nflag = nls(nflag);
