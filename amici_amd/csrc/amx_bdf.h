@@ -15,8 +15,12 @@
 //   quadratures (cvQuadNls)      cvodes.c:7089;  staggered FSA (cvStgrNls) :7167,
 //                                cvodes_nls_stg.c (residual / lsolve / conv test)
 // Dense LU with partial pivoting replaces SUNDlsMat_denseGETRF/GETRS
-// (sundials_dense.c:82-148): rows stay in the registers of their lane and the
-// pivot order is kept as a permutation instead of swapping rows.
+// (sundials_dense.c:82-148): row i of the Newton matrix belongs to lane i, is
+// factorised in that lane's registers and kept in LDS between setup and
+// solves; the pivot order is a permutation instead of row swaps.
+// Storage: the Nordsieck history, the LU factors and the method coefficient
+// arrays of an instance live in LDS (BdfLds); scalars and per-lane work values
+// live in registers.  Every function is inlined into the batch kernels.
 // Not modelled: BDF stability-limit detection (cvBDFStab), hmin/hmax limits.
 #pragma once
 #include "amx_team.h"
@@ -71,7 +75,7 @@ __device__ __forceinline__ double powr(double b, double e) { return pow(b, e); }
 
 // a[i] for a runtime index i
 template <int N>
-__device__ double sel(const double (&a)[N], int i) {
+__device__ __forceinline__ double sel(const double (&a)[N], int i) {
   double r = a[0];
 #pragma unroll
   for (int k = 1; k < N; ++k)
@@ -79,26 +83,49 @@ __device__ double sel(const double (&a)[N], int i) {
   return r;
 }
 
+// LDS block of one integrator instance.
+template <int NX, int NQ>
+struct BdfLds {
+  static constexpr int NQP = NQ > 0 ? NQ : 1;
+  double zn[LMAX][NX];    // Nordsieck history of the states (column i: lane i)
+  double znQ[LMAX][NQP];  // Nordsieck history of the quadratures
+  double lu[NX][NX];      // LU factors: (row i, column j) at lu[j][i]
+  double tau[LMAX + 1];   // previous step sizes
+  double l[LMAX];         // method coefficients
+  double tq[6];           // error-test constants
+  int perm[NX];           // perm[k] = lane of the k-th pivot row
+};
+
 // Row-per-lane dense LU of an NX x NX matrix held by the first NX team lanes.
+// The factors are kept in LDS (BdfLds::lu, perm); the per-lane pivot
+// bookkeeping lives in registers.
 template <int NX, int T>
 struct LURows {
-  double a[NX];
-  double dinv;   // 1 / pivot of the row this lane holds
-  int pos;       // elimination step at which this lane's row was the pivot
-  int src;       // lane that holds pivot row number `lane`
-  int perm[NX];  // perm[k] = lane of the k-th pivot row (team-uniform)
+  double dinv;  // 1 / pivot of the row this lane holds
+  int pos;      // elimination step at which this lane's row was the pivot
+  int src;      // lane that holds pivot row number `lane`
 
-  __device__ bool factor(const Team<T>& tm) {
-    pos = (tm.lane < NX) ? -1 : NX;
+  // Factorises the rows a[] (lane i: row i) and stores the factors in S.
+  // Returns false for a zero or non-finite pivot (recoverable setup failure).
+  template <class L>
+  __device__ __forceinline__ bool factor(const Team<T>& tm, double (&a)[NX], L& S) {
+    const bool act = tm.lane < NX;
+    pos = act ? -1 : NX;
     src = 0;
     dinv = 0.0;
+    bool ok = true;
 #pragma unroll
     for (int k = 0; k < NX; ++k) {
+      if (!ok) continue;  // (team-uniform; keeps the loop fully unrolled)
       double v = (pos < 0) ? fabs(a[k]) : -1.0;
+      if (v != v) v = INFINITY;  // a NaN entry fails the factorisation
       int p;
       const double mv = tm.argmax(v, p);
-      if (!(mv > 0.0)) return false;  // zero pivot: recoverable setup failure
-      perm[k] = p;
+      if (!(mv > 0.0) || !(mv < INFINITY)) {
+        ok = false;
+        continue;
+      }
+      S.perm[k] = p;
       if (tm.lane == k) src = p;
       double prow[NX];
 #pragma unroll
@@ -112,21 +139,28 @@ struct LURows {
         for (int j = k + 1; j < NX; ++j) a[j] = fma(-prow[j], l, a[j]);
       }
     }
-    return true;
+    if (act) {
+#pragma unroll
+      for (int j = 0; j < NX; ++j) S.lu[j][tm.lane] = a[j];
+    }
+    return ok;
   }
 
   // Solves (LU) x = b; b is this lane's component of the right-hand side.
-  __device__ double solve(double b, const Team<T>& tm) const {
+  template <class L>
+  __device__ __forceinline__ double solve(double b, const Team<T>& tm, const L& S) const {
+    const bool act = tm.lane < NX;
+    const int cl = act ? tm.lane : NX - 1;
 #pragma unroll
     for (int k = 0; k < NX; ++k) {
-      const double yk = tm.bcast(b, perm[k]);
-      if (pos > k) b = fma(-a[k], yk, b);
+      const double yk = tm.bcast(b, S.perm[k]);
+      if (act && pos > k) b = fma(-S.lu[k][cl], yk, b);
     }
 #pragma unroll
     for (int k = NX - 1; k >= 0; --k) {
-      const double xk = tm.bcast(b * dinv, perm[k]);
+      const double xk = tm.bcast(b * dinv, S.perm[k]);
       if (pos == k) b = xk;
-      else if (pos < k) b = fma(-a[k], xk, b);
+      else if (act && pos < k) b = fma(-S.lu[k][cl], xk, b);
     }
     return tm.gather(b, src);
   }
@@ -143,13 +177,15 @@ struct SolverParams {
   unsigned long long deadline;  // wall_clock64() limit, 0 = none
 };
 
+
 // BDF integrator over a Problem P.  P provides:
 //   NX, T, NQ (quadratures, 0 if none), NS (forward sensitivities, 0 if none)
 //   double rhs(double t, double y)                    team-collective, returns f_lane
-//   bool   quad_rhs(double t, double y, double* fq)   fq[EPQ] (if NQ > 0)
+//   void   quad_rhs(double t, double y, double* fq)   fq[EPQ] (if NQ > 0)
 //   void   jac(double t, double y)                    assembles J into its LDS
 //   void   build_M(double gamma, double* a)           a[j] = (I - gamma*J)[lane][j]
-//   sensitivity hooks (if NS > 0): see sens_* below
+//   bool   on_step(t, zn0, zn1, hscale)               after every accepted step
+//   sensitivity hooks (if NS > 0): see sens_* in the problem
 template <class P>
 struct BDF {
   static constexpr int NX = P::NX;
@@ -158,24 +194,24 @@ struct BDF {
   static constexpr int EPQ = (NQ > 0) ? (NQ + T - 1) / T : 1;
   static constexpr bool QUAD = NQ > 0;
   static constexpr bool SENS = P::NS > 0;
+  using L = BdfLds<NX, NQ>;
 
   P& pb;
+  L& S;
   Team<T> tm;
   SolverParams sp;
   bool active;  // lane < NX
 
-  // per-lane components
-  double zn[LMAX];
-  double ewt, acor, y, ftemp, tempv;
-  double znQ[LMAX][EPQ], ewtQ[EPQ], acorQ[EPQ], yQ[EPQ];
+  // per-lane work values
+  double ewt, acor, y, ftemp;
+  double ewtQ[EPQ], acorQ[EPQ], yQ[EPQ];
   LURows<NX, T> lu;
 
   // team-uniform scalars
   double tn, h, hprime, hscale, eta, etamax, etaq, etaqm1, etaqp1, gamma,
       gammap, gamrat, rl1, crate, crateS, delp, acnrm, acnrmS, saved_tq5, hu,
       tolsf, tstop, dsm_last, dsm_test;
-  double tau[LMAX + 1], l[LMAX], tq[6];
-  int q, qprime, L, qwait, nst, nstlp, nstlj, nscon, convfail;
+  int q, qprime, L_, qwait, nst, nstlp, nstlj, nscon, convfail;
   bool jcur, tstopset;
   // counters
   int nfe, nsetups, nje, nni, netf, ncfn, nfQe, nniS;
@@ -183,15 +219,35 @@ struct BDF {
   double* trace;
   int trace_cap, trace_n;
 
-  __device__ BDF(P& p, const SolverParams& s) : pb(p), sp(s) {
+  __device__ __forceinline__ BDF(P& p, L& lds, const SolverParams& s) : pb(p), S(lds), sp(s) {
     active = tm.lane < NX;
     trace = nullptr;
     trace_cap = 0;
     trace_n = 0;
   }
 
+  // ---------------- LDS accessors ----------------
+  // Nordsieck column j of this lane's state (0 on lanes >= NX).
+  __device__ __forceinline__ double Z(int j) const {
+    const double v = S.zn[j][active ? tm.lane : 0];
+    return active ? v : 0.0;
+  }
+  __device__ __forceinline__ void setZ(int j, double v) {
+    if (active) S.zn[j][tm.lane] = v;
+  }
+  __device__ __forceinline__ int qslot(int e) const { return tm.lane + e * T; }
+  __device__ __forceinline__ double ZQ(int j, int e) const {
+    const int g = qslot(e);
+    const double v = S.znQ[j][g < NQ ? g : 0];
+    return g < NQ ? v : 0.0;
+  }
+  __device__ __forceinline__ void setZQ(int j, int e, double v) {
+    const int g = qslot(e);
+    if (g < NQ) S.znQ[j][g] = v;
+  }
+
   // event: 1 accepted step, 2 nonlinear-solver failure, 3 error-test failure
-  __device__ void record(int event, double v) {
+  __device__ __forceinline__ void record(int event, double v) {
     if (trace == nullptr || trace_n >= trace_cap) return;
     if (tm.lane == 0) {
       double* r = trace + (size_t)trace_n * TRACE_REC;
@@ -202,108 +258,129 @@ struct BDF {
   }
 
   // ---------------- vector helpers ----------------
-  __device__ double wrms(double v, double w) const {
+  __device__ __forceinline__ double wrms(double v, double w) const {
     const double e = active ? v * w : 0.0;
     return sqrt(tm.sum(e * e) / NX);
   }
-  __device__ double wrmsQ(const double (&v)[EPQ], const double (&w)[EPQ]) const {
+  __device__ __forceinline__ double wrmsQ(const double (&v)[EPQ], const double (&w)[EPQ]) const {
     double acc = 0.0;
 #pragma unroll
     for (int k = 0; k < EPQ; ++k) {
-      const int g = tm.lane + k * T;
-      const double e = (g < NQ) ? v[k] * w[k] : 0.0;
+      const double e = (qslot(k) < NQ) ? v[k] * w[k] : 0.0;
       acc += e * e;
     }
     return sqrt(tm.sum(acc) / (NQ > 0 ? NQ : 1));
   }
-  __device__ void set_ewt() { ewt = active ? 1.0 / (sp.rtol * fabs(zn[0]) + sp.atol) : 0.0; }
-  __device__ void set_ewtQ() {
-#pragma unroll
-    for (int k = 0; k < EPQ; ++k) {
-      const int g = tm.lane + k * T;
-      ewtQ[k] = (g < NQ) ? 1.0 / (sp.rtolQ * fabs(znQ[0][k]) + sp.atolQ) : 0.0;
-    }
+  // true when any lane holds a non-finite value (NaN-safe, team-uniform)
+  __device__ __forceinline__ bool any_nonfinite(double v) const {
+    return tm.maxi((active && !isfinite(v)) ? 1 : 0) != 0;
   }
-  __device__ bool quad_errcon() const { return QUAD && sp.errconQ; }
+  __device__ __forceinline__ void set_ewt() {
+    ewt = active ? 1.0 / (sp.rtol * fabs(Z(0)) + sp.atol) : 0.0;
+  }
+  __device__ __forceinline__ void set_ewtQ() {
+#pragma unroll
+    for (int k = 0; k < EPQ; ++k)
+      ewtQ[k] = (qslot(k) < NQ) ? 1.0 / (sp.rtolQ * fabs(ZQ(0, k)) + sp.atolQ) : 0.0;
+  }
+  __device__ __forceinline__ bool quad_errcon() const { return QUAD && sp.errconQ; }
+  __device__ __forceinline__ void znQ0(double (&v)[EPQ]) const {
+#pragma unroll
+    for (int e = 0; e < EPQ; ++e) v[e] = ZQ(0, e);
+  }
 
   // ---------------- initialisation ----------------
-  // CVodeInit state (cvodes.c:677-886) for y0 on this lane, quadratures zero.
-  __device__ void init(double t0, double y0, const double* q0) {
+  // CVodeInit state (cvodes.c:677-886) for y0 on this lane, quadratures q0.
+  __device__ __forceinline__ void init(double t0, double y0, const double* q0) {
     tn = t0;
-#pragma unroll
-    for (int j = 0; j < LMAX; ++j) zn[j] = 0.0;
-    zn[0] = active ? y0 : 0.0;
+    team_sync();
+    for (int j = 0; j < LMAX; ++j) setZ(j, 0.0);
+    setZ(0, y0);
     if constexpr (QUAD) {
-#pragma unroll
       for (int j = 0; j < LMAX; ++j)
 #pragma unroll
-        for (int k = 0; k < EPQ; ++k) znQ[j][k] = 0.0;
+        for (int e = 0; e < EPQ; ++e) setZQ(j, e, 0.0);
 #pragma unroll
-      for (int k = 0; k < EPQ; ++k) znQ[0][k] = q0 ? q0[k] : 0.0;
+      for (int e = 0; e < EPQ; ++e) setZQ(0, e, q0 ? q0[e] : 0.0);
     }
-    q = 1; L = 2; qwait = L; etamax = ETA_MAX_FS;
+    if (tm.lane == 0) {
+      for (int j = 0; j <= LMAX; ++j) S.tau[j] = 0.0;
+      for (int j = 0; j < LMAX; ++j) S.l[j] = 0.0;
+      for (int j = 0; j < 6; ++j) S.tq[j] = 0.0;
+    }
+    team_sync();
+    q = 1; L_ = 2; qwait = L_; qprime = 1; etamax = ETA_MAX_FS;
     hu = 0.0; tolsf = 1.0; h = 0.0; hprime = 0.0; hscale = 0.0; eta = 1.0;
     nst = 0; nstlp = 0; nstlj = 0; nscon = 0; saved_tq5 = 0.0;
     gammap = 0.0; gamma = 0.0; gamrat = 1.0; crate = 1.0; crateS = 1.0;
     delp = 0.0; acnrm = 0.0; acnrmS = 0.0; convfail = NO_FAILURES;
-    jcur = false; tstopset = false; etaq = etaqm1 = etaqp1 = 0.0;
-#pragma unroll
-    for (int j = 0; j <= LMAX; ++j) tau[j] = 0.0;
-#pragma unroll
-    for (int j = 0; j < LMAX; ++j) l[j] = 0.0;
-#pragma unroll
-    for (int j = 0; j < 6; ++j) tq[j] = 0.0;
+    jcur = false; tstopset = false; tstop = 0.0; etaq = etaqm1 = etaqp1 = 0.0;
     nfe = nsetups = nje = nni = netf = ncfn = nfQe = nniS = 0;
-    dsm_last = 0.0;
+    dsm_last = 0.0; dsm_test = 0.0;
+    ewt = 0.0; acor = 0.0; y = active ? y0 : 0.0; ftemp = 0.0;
+#pragma unroll
+    for (int e = 0; e < EPQ; ++e) ewtQ[e] = acorQ[e] = yQ[e] = 0.0;
   }
 
   // ---------------- Nordsieck array manipulation ----------------
-  template <class F>
-  __device__ void for_zn(F&& f) {  // f(j, double& znj_state, double (&znQj)[EPQ])
+  // zn[j-1] += zn[j] for k = 1..q, j = q..k (cvPredict)
+  __device__ __forceinline__ void pascal(double sign) {
+    if (active) {
+      double z[LMAX];
 #pragma unroll
-    for (int j = 0; j < LMAX; ++j) f(j, zn[j], znQ[j]);
+      for (int j = 0; j < LMAX; ++j) z[j] = (j <= q) ? S.zn[j][tm.lane] : 0.0;
+#pragma unroll
+      for (int k = 1; k <= QMAX; ++k)
+#pragma unroll
+        for (int j = QMAX; j >= k; --j)
+          if (k <= q && j <= q) z[j - 1] = fma(sign, z[j], z[j - 1]);
+#pragma unroll
+      for (int j = 0; j < QMAX; ++j)
+        if (j < q) S.zn[j][tm.lane] = z[j];
+    }
+    if constexpr (QUAD) {
+#pragma unroll
+      for (int e = 0; e < EPQ; ++e) {
+        const int g = qslot(e);
+        if (g < NQ) {
+          double z[LMAX];
+#pragma unroll
+          for (int j = 0; j < LMAX; ++j) z[j] = (j <= q) ? S.znQ[j][g] : 0.0;
+#pragma unroll
+          for (int k = 1; k <= QMAX; ++k)
+#pragma unroll
+            for (int j = QMAX; j >= k; --j)
+              if (k <= q && j <= q) z[j - 1] = fma(sign, z[j], z[j - 1]);
+#pragma unroll
+          for (int j = 0; j < QMAX; ++j)
+            if (j < q) S.znQ[j][g] = z[j];
+        }
+      }
+    }
   }
 
-  __device__ void predict() {
+  __device__ __forceinline__ void predict() {
     tn += h;
     if (tstopset && (tn - tstop) * h > 0.0) tn = tstop;
-#pragma unroll
-    for (int k = 1; k <= QMAX; ++k)
-#pragma unroll
-      for (int j = QMAX; j >= k; --j)
-        if (k <= q && j <= q) {
-          zn[j - 1] += zn[j];
-          if constexpr (QUAD)
-#pragma unroll
-            for (int e = 0; e < EPQ; ++e) znQ[j - 1][e] += znQ[j][e];
-        }
+    pascal(1.0);
     if constexpr (SENS) pb.sens_predict(q);
   }
 
-  __device__ void restore(double saved_t) {
+  __device__ __forceinline__ void restore(double saved_t) {
     tn = saved_t;
-#pragma unroll
-    for (int k = 1; k <= QMAX; ++k)
-#pragma unroll
-      for (int j = QMAX; j >= k; --j)
-        if (k <= q && j <= q) {
-          zn[j - 1] -= zn[j];
-          if constexpr (QUAD)
-#pragma unroll
-            for (int e = 0; e < EPQ; ++e) znQ[j - 1][e] -= znQ[j][e];
-        }
+    pascal(-1.0);
     if constexpr (SENS) pb.sens_restore(q);
   }
 
-  __device__ void rescale() {
+  __device__ __forceinline__ void rescale() {
     double c = eta;
 #pragma unroll
     for (int j = 1; j <= QMAX; ++j) {
       if (j <= q) {
-        zn[j] *= c;
+        setZ(j, Z(j) * c);
         if constexpr (QUAD)
 #pragma unroll
-          for (int e = 0; e < EPQ; ++e) znQ[j][e] *= c;
+          for (int e = 0; e < EPQ; ++e) setZQ(j, e, ZQ(j, e) * c);
       }
       c *= eta;
     }
@@ -313,8 +390,8 @@ struct BDF {
     nscon = 0;
   }
 
-  // Adds coef[j] * zn[src] to zn[j] for 2 <= j <= jmax.
-  __device__ void increase_bdf() {
+  // cvIncreaseBDF: zn[q+1] = A1 zn[QMAX] (saved acor), zn[j] += l[j] zn[q+1]
+  __device__ __forceinline__ void increase_bdf() {
     double ll[LMAX];
 #pragma unroll
     for (int i = 0; i < LMAX; ++i) ll[i] = 0.0;
@@ -323,7 +400,7 @@ struct BDF {
 #pragma unroll
     for (int j = 1; j < QMAX; ++j) {
       if (j < q) {
-        hsum += tau[j + 1];
+        hsum += S.tau[j + 1];
         const double xi = hsum / hscale;
         prod *= xi;
         alpha0 -= 1.0 / (j + 1);
@@ -335,30 +412,26 @@ struct BDF {
       }
     }
     const double A1 = (-alpha0 - alpha1) / prod;
-    // zn[L] = A1 * zn[QMAX] (saved acor);  zn[j] += l[j] * zn[L], j = 2..q
-    double znL = A1 * zn[QMAX];
+    const double znL = A1 * Z(QMAX);
 #pragma unroll
     for (int j = 2; j <= QMAX; ++j)
-      if (j <= q) zn[j] = fma(ll[j], znL, zn[j]);
-#pragma unroll
-    for (int j = 2; j <= QMAX; ++j)
-      if (j == q + 1) zn[j] = znL;
+      if (j <= q) setZ(j, fma(ll[j], znL, Z(j)));
+    if (q + 1 <= QMAX) setZ(q + 1, znL);
     if constexpr (QUAD) {
 #pragma unroll
       for (int e = 0; e < EPQ; ++e) {
-        double zQL = A1 * znQ[QMAX][e];
+        const double zQL = A1 * ZQ(QMAX, e);
 #pragma unroll
         for (int j = 2; j <= QMAX; ++j)
-          if (j <= q) znQ[j][e] = fma(ll[j], zQL, znQ[j][e]);
-#pragma unroll
-        for (int j = 2; j <= QMAX; ++j)
-          if (j == q + 1) znQ[j][e] = zQL;
+          if (j <= q) setZQ(j, e, fma(ll[j], zQL, ZQ(j, e)));
+        if (q + 1 <= QMAX) setZQ(q + 1, e, zQL);
       }
     }
     if constexpr (SENS) pb.sens_increase(q, A1, ll);
   }
 
-  __device__ void decrease_bdf() {
+  // cvDecreaseBDF
+  __device__ __forceinline__ void decrease_bdf() {
     double ll[LMAX];
 #pragma unroll
     for (int i = 0; i < LMAX; ++i) ll[i] = 0.0;
@@ -367,61 +440,62 @@ struct BDF {
 #pragma unroll
     for (int j = 1; j <= QMAX - 2; ++j) {
       if (j <= q - 2) {
-        hsum += tau[j];
+        hsum += S.tau[j];
         const double xi = hsum / hscale;
 #pragma unroll
         for (int i = j + 2; i >= 2; --i) ll[i] = ll[i] * xi + ll[i - 1];
       }
     }
     if (q > 2) {
-      const double znq = sel(zn, q);
+      const double znq = Z(q);
 #pragma unroll
       for (int j = 2; j < QMAX; ++j)
-        if (j < q) zn[j] = fma(-ll[j], znq, zn[j]);
+        if (j < q) setZ(j, fma(-ll[j], znq, Z(j)));
       if constexpr (QUAD) {
 #pragma unroll
         for (int e = 0; e < EPQ; ++e) {
-          double zq = znQ[0][e];
-#pragma unroll
-          for (int j = 1; j < LMAX; ++j)
-            if (j == q) zq = znQ[j][e];
+          const double zq = ZQ(q, e);
 #pragma unroll
           for (int j = 2; j < QMAX; ++j)
-            if (j < q) znQ[j][e] = fma(-ll[j], zq, znQ[j][e]);
+            if (j < q) setZQ(j, e, fma(-ll[j], zq, ZQ(j, e)));
         }
       }
       if constexpr (SENS) pb.sens_decrease(q, ll);
     }
   }
 
-  __device__ void adjust_order(int deltaq) {
+  __device__ __forceinline__ void adjust_order(int deltaq) {
     if (q == 2 && deltaq != 1) return;
     if (deltaq == 1) increase_bdf();
     else if (deltaq == -1) decrease_bdf();
   }
 
-  __device__ void adjust_params() {
+  __device__ __forceinline__ void adjust_params() {
     if (qprime != q) {
       adjust_order(qprime - q);
       q = qprime;
-      L = q + 1;
-      qwait = L;
+      L_ = q + 1;
+      qwait = L_;
     }
     rescale();
   }
 
-  // ---------------- method coefficients ----------------
-  __device__ void set_bdf() {
+  // ---------------- method coefficients: cvSetBDF / cvSetTqBDF ----------------
+  __device__ __forceinline__ void set_coeffs() {
+    double l[LMAX], tq[6];
     double xi_inv = 1.0, xistar_inv = 1.0, alpha0 = -1.0, alpha0_hat = -1.0;
     double hsum = h;
 #pragma unroll
     for (int i = 0; i < LMAX; ++i) l[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) tq[i] = S.tq[i];
     l[0] = l[1] = 1.0;
+    double lq = 1.0;
     if (q > 1) {
 #pragma unroll
       for (int j = 2; j < QMAX; ++j) {
         if (j < q) {
-          hsum += tau[j - 1];
+          hsum += S.tau[j - 1];
           xi_inv = h / hsum;
           alpha0 -= 1.0 / j;
 #pragma unroll
@@ -430,21 +504,23 @@ struct BDF {
       }
       alpha0 -= 1.0 / q;
       xistar_inv = -l[1] - alpha0;
-      hsum += sel(tau, q - 1);
+      hsum += S.tau[q - 1];
       xi_inv = h / hsum;
       alpha0_hat = -l[1] - xi_inv;
 #pragma unroll
       for (int i = QMAX; i >= 1; --i)
         if (i <= q) l[i] += l[i - 1] * xistar_inv;
+#pragma unroll
+      for (int i = 2; i <= QMAX; ++i)
+        if (i == q) lq = l[i];
     }
-    // cvSetTqBDF
     const double A1 = 1.0 - alpha0_hat + alpha0;
     const double A2 = 1.0 + q * A1;
     tq[2] = fabs(A1 / (alpha0 * A2));
-    tq[5] = fabs(A2 * xistar_inv / (sel(l, q) * xi_inv));
+    tq[5] = fabs(A2 * xistar_inv / (lq * xi_inv));
     if (qwait == 1) {
       if (q > 1) {
-        const double C = xistar_inv / sel(l, q);
+        const double C = xistar_inv / lq;
         const double A3 = alpha0 + 1.0 / q;
         const double A4 = alpha0_hat + xi_inv;
         const double Cpinv = (1.0 - A4 + A3) / A3;
@@ -452,7 +528,7 @@ struct BDF {
       } else {
         tq[1] = 1.0;
       }
-      hsum += sel(tau, q);
+      hsum += S.tau[q];
       xi_inv = h / hsum;
       const double A5 = alpha0 - 1.0 / (q + 1);
       const double A6 = alpha0_hat - xi_inv;
@@ -460,10 +536,14 @@ struct BDF {
       tq[3] = fabs(Cppinv / (xi_inv * (q + 2) * A5));
     }
     tq[4] = CORTES / tq[2];
-  }
-
-  __device__ void set_coeffs() {
-    set_bdf();
+    team_sync();
+    if (tm.lane == 0) {
+#pragma unroll
+      for (int i = 0; i < LMAX; ++i) S.l[i] = l[i];
+#pragma unroll
+      for (int i = 1; i < 6; ++i) S.tq[i] = tq[i];
+    }
+    team_sync();
     rl1 = 1.0 / l[1];
     gamma = h * rl1;
     if (nst == 0) gammap = gamma;
@@ -472,7 +552,7 @@ struct BDF {
 
   // ---------------- linear system ----------------
   // cvLsSetup + cvNlsLSetup; returns NLS_OK or NLS_RECOVER (singular).
-  __device__ int lsetup(int cf) {
+  __device__ __forceinline__ int lsetup(int cf) {
     const double dgamma = fabs(gamma / gammap - 1.0);
     const bool jbad = (nst == 0) || (nst >= nstlj + MSBJ) ||
                       (cf == FAIL_BAD_J && dgamma < DGMAX_JBAD) || (cf == FAIL_OTHER);
@@ -484,8 +564,9 @@ struct BDF {
     } else {
       jcur = false;
     }
-    pb.build_M(gamma, lu.a);
-    const bool ok = lu.factor(tm);
+    double a[NX];
+    pb.build_M(gamma, a);
+    const bool ok = lu.factor(tm, a, S);
     nsetups++;
     gamrat = 1.0;
     gammap = gamma;
@@ -495,42 +576,45 @@ struct BDF {
     return ok ? NLS_OK : NLS_RECOVER;
   }
 
-  __device__ double lsolve(double b) const {
-    double x = lu.solve(b, tm);
+  __device__ __forceinline__ double lsolve(double b) const {
+    double x = lu.solve(b, tm, S);
     if (gamrat != 1.0) x *= 2.0 / (1.0 + gamrat);
     return x;
   }
 
   // ---------------- nonlinear solve (Newton) ----------------
   // cvNlsResidual: y = zn0 + acor, ftemp = f(tn, y), returns residual.
-  __device__ double residual() {
-    y = zn[0] + acor;
+  __device__ __forceinline__ double residual() {
+    y = Z(0) + acor;
     ftemp = pb.rhs(tn, y);
     nfe++;
-    return rl1 * zn[1] + acor - gamma * ftemp;
+    return rl1 * Z(1) + acor - gamma * ftemp;
   }
 
-  // SUNNonlinSolSolve_Newton with cvNlsConvTest; returns NLS_OK / NLS_RECOVER / <0.
-  __device__ int newton(bool callSetup) {
+  // SUNNonlinSolSolve_Newton with cvNlsConvTest; returns NLS_OK / NLS_RECOVER.
+  // A non-finite residual or correction is a recoverable failure; the number
+  // of fresh-Jacobian retries is bounded (CVODES needs at most two).
+  __device__ __forceinline__ int newton(bool callSetup) {
     bool jbad = false;
-    for (int attempt = 0;; ++attempt) {
+    for (int attempt = 0; attempt < 3; ++attempt) {
       int m = 0;
       double res = residual();
       int ret = NLS_OK;
-      if (!isfinite(tm.max(active ? fabs(res) : 0.0))) ret = NLS_RECOVER;
+      if (any_nonfinite(res)) ret = NLS_RECOVER;
       if (ret == NLS_OK && callSetup) {
         if (jbad) convfail = FAIL_BAD_J;
         ret = lsetup(convfail);
       }
       if (ret == NLS_OK) {
-        for (;;) {
+        for (int it = 0; it < NLS_MAXCOR; ++it) {
           nni++;
           double delta = lsolve(-res);
           if (!active) delta = 0.0;
           acor += delta;
           const double del = wrms(delta, ewt);
+          if (!(del == del)) { ret = NLS_RECOVER; break; }
           if (m > 0) crate = fmax(CRDOWN * crate, del / delp);
-          const double dcon = del * fmin(1.0, crate) / tq[4];
+          const double dcon = del * fmin(1.0, crate) / S.tq[4];
           if (dcon <= 1.0) {
             acnrm = (m == 0) ? del : wrms(acor, ewt);
             jcur = false;
@@ -541,12 +625,10 @@ struct BDF {
           m++;
           if (m >= NLS_MAXCOR) { ret = NLS_RECOVER; break; }
           res = residual();
-          if (!isfinite(tm.max(active ? fabs(res) : 0.0))) { ret = NLS_RECOVER; break; }
+          if (any_nonfinite(res)) { ret = NLS_RECOVER; break; }
         }
       }
-      // (CVODES retries with a fresh Jacobian while it is not current; the
-      // attempt cap only guards against non-finite step data)
-      if (ret == NLS_RECOVER && !jcur && attempt < 3) {
+      if (ret == NLS_RECOVER && !jcur) {
         callSetup = true;
         jbad = true;
         acor = 0.0;
@@ -554,9 +636,10 @@ struct BDF {
       }
       return ret;
     }
+    return NLS_RECOVER;
   }
 
-  __device__ int nls(int nflag) {
+  __device__ __forceinline__ int nls(int nflag) {
     convfail = (nflag == FIRST_CALL || nflag == PREV_ERR_FAIL) ? NO_FAILURES : FAIL_OTHER;
     const bool callSetup = (nflag == PREV_CONV_FAIL) || (nflag == PREV_ERR_FAIL) ||
                            (nst == 0) || (nst >= nstlp + MSBP) ||
@@ -564,13 +647,13 @@ struct BDF {
     acor = 0.0;
     const int flag = newton(callSetup);
     if (flag != NLS_OK) return flag;
-    y = zn[0] + acor;
+    y = Z(0) + acor;
     jcur = false;
     return NLS_OK;
   }
 
   // cvHandleNFlag
-  __device__ int handle_nflag(int& nflag, double saved_t, int& ncf) {
+  __device__ __forceinline__ int handle_nflag(int& nflag, double saved_t, int& ncf) {
     if (nflag == NLS_OK) return DO_ERROR_TEST;
     record(2, (double)nflag);
     ncfn++;
@@ -584,10 +667,10 @@ struct BDF {
     return PREDICT_AGAIN;
   }
 
-  // cvDoErrorTest (state part); extra(...) re-evaluates quadrature / sensitivity
-  // derivatives on the order-1 restart branch.
-  __device__ int error_test(int& nflag, double saved_t, double acor_nrm, int& nef) {
-    const double dsm = acor_nrm * tq[2];
+  // cvDoErrorTest (state part); the order-1 restart branch re-evaluates the
+  // quadrature / sensitivity derivatives.
+  __device__ __forceinline__ int error_test(int& nflag, double saved_t, double acor_nrm, int& nef) {
+    const double dsm = acor_nrm * S.tq[2];
     dsm_test = dsm;
     if (dsm <= 1.0) return AMX_SUCCESS;
     record(3, dsm);
@@ -598,7 +681,7 @@ struct BDF {
     if (nef == MXNEF) return AMX_ERR_FAILURE;
     etamax = 1.0;
     if (nef <= MXNEF1) {
-      eta = 1.0 / (powr(BIAS2 * dsm, 1.0 / L) + ADDON);
+      eta = 1.0 / (powr(BIAS2 * dsm, 1.0 / L_) + ADDON);
       eta = fmax(ETA_MIN_EF, eta);
       if (nef >= SMALL_NEF) eta = fmin(eta, ETA_MAX_EF);
       rescale();
@@ -607,9 +690,9 @@ struct BDF {
     if (q > 1) {
       eta = ETA_MIN_EF;
       adjust_order(-1);
-      L = q;
+      L_ = q;
       q--;
-      qwait = L;
+      qwait = L_;
       rescale();
       return TRY_AGAIN;
     }
@@ -618,63 +701,76 @@ struct BDF {
     hscale = h;
     qwait = LONG_WAIT;
     nscon = 0;
-    const double f0 = pb.rhs(tn, zn[0]);
+    const double f0 = pb.rhs(tn, Z(0));
     nfe++;
-    zn[1] = active ? h * f0 : 0.0;
+    setZ(1, h * f0);
     if constexpr (QUAD) {
       double fq[EPQ];
-      pb.quad_rhs(tn, zn[0], fq);
+      pb.quad_rhs(tn, Z(0), fq);
       nfQe++;
 #pragma unroll
-      for (int e = 0; e < EPQ; ++e) znQ[1][e] = h * fq[e];
+      for (int e = 0; e < EPQ; ++e) setZQ(1, e, h * fq[e]);
     }
-    if constexpr (SENS) pb.sens_restart_q1(tn, zn[0], f0, h);
+    if constexpr (SENS) pb.sens_restart_q1(tn, Z(0), f0, h);
     return TRY_AGAIN;
   }
 
   // cvQuadNls
-  __device__ void quad_correct() {
+  __device__ __forceinline__ void quad_correct() {
     double fq[EPQ];
     pb.quad_rhs(tn, y, fq);
     nfQe++;
 #pragma unroll
     for (int e = 0; e < EPQ; ++e) {
-      acorQ[e] = rl1 * (h * fq[e] - znQ[1][e]);
-      yQ[e] = znQ[0][e] + acorQ[e];
+      acorQ[e] = rl1 * (h * fq[e] - ZQ(1, e));
+      yQ[e] = ZQ(0, e) + acorQ[e];
     }
   }
 
   // cvCompleteStep
-  __device__ void complete_step() {
+  __device__ __forceinline__ void complete_step() {
     nst++;
     nscon++;
     hu = h;
+    double tau[LMAX + 1];
+#pragma unroll
+    for (int i = 0; i <= LMAX; ++i) tau[i] = S.tau[i];
 #pragma unroll
     for (int i = QMAX; i >= 2; --i)
       if (i <= q) tau[i] = tau[i - 1];
     if (q == 1 && nst > 1) tau[2] = tau[1];
     tau[1] = h;
+    team_sync();
+    if (tm.lane == 0) {
 #pragma unroll
-    for (int j = 0; j <= QMAX; ++j)
-      if (j <= q) {
-        zn[j] = fma(l[j], acor, zn[j]);
-        if constexpr (QUAD)
-#pragma unroll
-          for (int e = 0; e < EPQ; ++e) znQ[j][e] = fma(l[j], acorQ[e], znQ[j][e]);
-      }
-    if constexpr (SENS) pb.sens_complete(q, l);
-    qwait--;
-    if (qwait == 1 && q != QMAX) {
-      zn[QMAX] = acor;
+      for (int i = 0; i <= LMAX; ++i) S.tau[i] = tau[i];
+    }
+    team_sync();
+    for (int j = 0; j <= q; ++j) {
+      const double lj = S.l[j];
+      setZ(j, fma(lj, acor, Z(j)));
       if constexpr (QUAD)
 #pragma unroll
-        for (int e = 0; e < EPQ; ++e) znQ[QMAX][e] = acorQ[e];
+        for (int e = 0; e < EPQ; ++e) setZQ(j, e, fma(lj, acorQ[e], ZQ(j, e)));
+    }
+    if constexpr (SENS) {
+      double l[LMAX];
+#pragma unroll
+      for (int j = 0; j < LMAX; ++j) l[j] = S.l[j];
+      pb.sens_complete(q, l);
+    }
+    qwait--;
+    if (qwait == 1 && q != QMAX) {
+      setZ(QMAX, acor);
+      if constexpr (QUAD)
+#pragma unroll
+        for (int e = 0; e < EPQ; ++e) setZQ(QMAX, e, acorQ[e]);
       if constexpr (SENS) pb.sens_save_acor();
-      saved_tq5 = tq[5];
+      saved_tq5 = S.tq[5];
     }
   }
 
-  __device__ void set_eta() {
+  __device__ __forceinline__ void set_eta() {
     if (eta > 0.0 && eta < ETA_MAX_FX) {
       eta = 1.0;
       hprime = h;
@@ -689,8 +785,8 @@ struct BDF {
     }
   }
 
-  __device__ double norm_with_extras(double nstate, const double (&vQ)[EPQ], bool useQ,
-                                     double sensnorm, bool useS) const {
+  __device__ __forceinline__ double norm_with_extras(double nstate, const double (&vQ)[EPQ],
+                                                     bool useQ, double sensnorm, bool useS) const {
     double r = nstate;
     if constexpr (QUAD)
       if (useQ) r = fmax(r, wrmsQ(vQ, ewtQ));
@@ -698,45 +794,40 @@ struct BDF {
     return r;
   }
 
-  __device__ double compute_etaqm1() {
+  __device__ __forceinline__ double compute_etaqm1() {
     etaqm1 = 0.0;
     if (q > 1) {
       double zq[EPQ];
 #pragma unroll
-      for (int e = 0; e < EPQ; ++e) {
-        zq[e] = znQ[0][e];
-#pragma unroll
-        for (int j = 1; j < LMAX; ++j)
-          if (j == q) zq[e] = znQ[j][e];
-      }
-      double ddn = wrms(sel(zn, q), ewt);
+      for (int e = 0; e < EPQ; ++e) zq[e] = ZQ(q, e);
+      double ddn = wrms(Z(q), ewt);
       double sn = 0.0;
       if constexpr (SENS) sn = pb.sens_norm_zn(q);
-      ddn = norm_with_extras(ddn, zq, quad_errcon(), sn, SENS) * tq[1];
+      ddn = norm_with_extras(ddn, zq, quad_errcon(), sn, SENS) * S.tq[1];
       etaqm1 = 1.0 / (powr(BIAS1 * ddn, 1.0 / q) + ADDON);
     }
     return etaqm1;
   }
 
-  __device__ double compute_etaqp1() {
+  __device__ __forceinline__ double compute_etaqp1() {
     etaqp1 = 0.0;
     if (q != QMAX) {
       if (saved_tq5 == 0.0) return etaqp1;
-      const double cquot = (tq[5] / saved_tq5) * powr(h / tau[2], (double)L);
-      const double tv = acor - cquot * zn[QMAX];
+      const double cquot = (S.tq[5] / saved_tq5) * powr(h / S.tau[2], (double)L_);
+      const double tv = acor - cquot * Z(QMAX);
       double tQ[EPQ];
 #pragma unroll
-      for (int e = 0; e < EPQ; ++e) tQ[e] = acorQ[e] - cquot * znQ[QMAX][e];
+      for (int e = 0; e < EPQ; ++e) tQ[e] = acorQ[e] - cquot * ZQ(QMAX, e);
       double dup = wrms(tv, ewt);
       double sn = 0.0;
       if constexpr (SENS) sn = pb.sens_norm_qp1(cquot);
-      dup = norm_with_extras(dup, tQ, quad_errcon(), sn, SENS) * tq[3];
-      etaqp1 = 1.0 / (powr(BIAS3 * dup, 1.0 / (L + 1)) + ADDON);
+      dup = norm_with_extras(dup, tQ, quad_errcon(), sn, SENS) * S.tq[3];
+      etaqp1 = 1.0 / (powr(BIAS3 * dup, 1.0 / (L_ + 1)) + ADDON);
     }
     return etaqp1;
   }
 
-  __device__ void choose_eta() {
+  __device__ __forceinline__ void choose_eta() {
     const double etam = fmax(etaqm1, fmax(etaq, etaqp1));
     if (etam > 0.0 && etam < ETA_MAX_FX) {
       eta = 1.0;
@@ -750,16 +841,16 @@ struct BDF {
     } else {
       eta = etaqp1;
       qprime = q + 1;
-      zn[QMAX] = acor;
+      setZ(QMAX, acor);
       if constexpr (QUAD)
         if (quad_errcon())
 #pragma unroll
-          for (int e = 0; e < EPQ; ++e) znQ[QMAX][e] = acorQ[e];
+          for (int e = 0; e < EPQ; ++e) setZQ(QMAX, e, acorQ[e]);
       if constexpr (SENS) pb.sens_save_acor();
     }
   }
 
-  __device__ void prepare_next_step(double dsm) {
+  __device__ __forceinline__ void prepare_next_step(double dsm) {
     if (etamax == 1.0) {
       qwait = max(qwait, 2);
       qprime = q;
@@ -767,7 +858,7 @@ struct BDF {
       eta = 1.0;
       return;
     }
-    etaq = 1.0 / (powr(BIAS2 * dsm, 1.0 / L) + ADDON);
+    etaq = 1.0 / (powr(BIAS2 * dsm, 1.0 / L_) + ADDON);
     if (qwait != 0) {
       eta = etaq;
       qprime = q;
@@ -782,13 +873,14 @@ struct BDF {
   }
 
   // ---------------- one step: cvStep ----------------
-  __device__ int step() {
+  __device__ __forceinline__ int step() {
     int ncf = 0, nef = 0, nefQ = 0, ncfS = 0, nefS = 0;
     if (nst > 0 && hprime != h) adjust_params();
     const double saved_t = tn;
     int nflag = FIRST_CALL;
     double dsm = 0.0;
-    for (int attempt = 0;; ++attempt) {
+    int attempt = 0;
+    for (;; ++attempt) {
       if (attempt >= MAX_ATTEMPTS) return AMX_CONV_FAILURE;
       predict();
       set_coeffs();
@@ -837,17 +929,16 @@ struct BDF {
   }
 
   // ---------------- initial step size: cvHin ----------------
-  __device__ double upper_bound_h0(double tdist) {
-    double t1 = active ? (HUB_FACTOR * fabs(zn[0]) + 1.0 / ewt) : 1.0;
-    double hub_inv = tm.max(active ? fabs(zn[1]) / t1 : 0.0);
+  __device__ __forceinline__ double upper_bound_h0(double tdist) {
+    const double t1 = active ? (HUB_FACTOR * fabs(Z(0)) + 1.0 / ewt) : 1.0;
+    double hub_inv = tm.max(active ? fabs(Z(1)) / t1 : 0.0);
     if constexpr (QUAD) {
       if (quad_errcon()) {
         double m = 0.0;
 #pragma unroll
-        for (int e = 0; e < EPQ; ++e) {
-          const int g = tm.lane + e * T;
-          if (g < NQ) m = fmax(m, fabs(znQ[1][e]) / (HUB_FACTOR * fabs(znQ[0][e]) + 1.0 / ewtQ[e]));
-        }
+        for (int e = 0; e < EPQ; ++e)
+          if (qslot(e) < NQ)
+            m = fmax(m, fabs(ZQ(1, e)) / (HUB_FACTOR * fabs(ZQ(0, e)) + 1.0 / ewtQ[e]));
         hub_inv = fmax(hub_inv, tm.max(m));
       }
     }
@@ -857,12 +948,12 @@ struct BDF {
     return hub;
   }
 
-  __device__ double ydd_norm(double hg) {
-    const double yy = zn[0] + hg * zn[1];
+  __device__ __forceinline__ double ydd_norm(double hg) {
+    const double yy = Z(0) + hg * Z(1);
     if constexpr (SENS) pb.sens_ydd_prepare(hg);
-    double fv = pb.rhs(tn + hg, yy);
+    const double fv = pb.rhs(tn + hg, yy);
     nfe++;
-    double tv = (fv - zn[1]) / hg;
+    const double tv = (fv - Z(1)) / hg;
     double r = wrms(tv, ewt);
     if constexpr (QUAD) {
       if (quad_errcon()) {
@@ -871,7 +962,7 @@ struct BDF {
         nfQe++;
         double tQ[EPQ];
 #pragma unroll
-        for (int e = 0; e < EPQ; ++e) tQ[e] = (fq[e] - znQ[1][e]) / hg;
+        for (int e = 0; e < EPQ; ++e) tQ[e] = (fq[e] - ZQ(1, e)) / hg;
         r = fmax(r, wrmsQ(tQ, ewtQ));
       }
     }
@@ -879,7 +970,7 @@ struct BDF {
     return r;
   }
 
-  __device__ int hin(double tout) {
+  __device__ __forceinline__ int hin(double tout) {
     const double tdiff = tout - tn;
     if (tdiff == 0.0) return AMX_TOO_CLOSE;
     const double sign = (tdiff > 0.0) ? 1.0 : -1.0;
@@ -926,36 +1017,37 @@ struct BDF {
   // Advances to tout; on success tn == tout and zn[0] holds y(tout).
   // pb.on_step() is invoked for the initial point and after every accepted
   // step (trajectory storage); returning false stops with AMX_TRAJ_FULL.
-  __device__ int advance(double tout) {
+  __device__ __forceinline__ int advance(double tout) {
     tstop = tout;
     tstopset = true;
     if (nst == 0) {
       set_ewt();
       if constexpr (QUAD) set_ewtQ();
       if constexpr (SENS) pb.sens_set_ewt();
-      const double f0 = pb.rhs(tn, zn[0]);
+      const double f0 = pb.rhs(tn, Z(0));
       nfe++;
-      zn[1] = active ? f0 : 0.0;
+      setZ(1, f0);
       if constexpr (QUAD) {
         double fq[EPQ];
-        pb.quad_rhs(tn, zn[0], fq);
+        pb.quad_rhs(tn, Z(0), fq);
         nfQe++;
 #pragma unroll
-        for (int e = 0; e < EPQ; ++e) znQ[1][e] = fq[e];
+        for (int e = 0; e < EPQ; ++e) setZQ(1, e, fq[e]);
       }
-      if constexpr (SENS) pb.sens_init_deriv(tn, zn[0], f0);
+      if constexpr (SENS) pb.sens_init_deriv(tn, Z(0), f0);
+      if (any_nonfinite(f0)) return AMX_RHSFUNC_FAIL;
       if ((tstop - tn) * (tout - tn) <= 0.0) return AMX_ILL_INPUT;
-      int hf = hin(tout);
+      const int hf = hin(tout);
       if (hf != AMX_SUCCESS) return hf;
       if ((tn + h - tstop) * h > 0.0) h = (tstop - tn) * (1.0 - 4.0 * UROUND);
       hscale = h;
       hprime = h;
-      zn[1] *= h;
+      setZ(1, Z(1) * h);
       if constexpr (QUAD)
 #pragma unroll
-        for (int e = 0; e < EPQ; ++e) znQ[1][e] *= h;
+        for (int e = 0; e < EPQ; ++e) setZQ(1, e, ZQ(1, e) * h);
       if constexpr (SENS) pb.sens_scale_deriv(h);
-      if (!pb.on_step(tn, zn[0], zn[1], hscale)) return AMX_TRAJ_FULL;
+      if (!pb.on_step(tn, Z(0), Z(1), hscale)) return AMX_TRAJ_FULL;
     } else {
       const double troundoff = FUZZ_FACTOR * UROUND * (fabs(tn) + fabs(h));
       if (fabs(tn - tstop) <= troundoff) {
@@ -975,9 +1067,16 @@ struct BDF {
       }
       if (nst >= sp.maxsteps) return AMX_TOO_MUCH_WORK;
       if (sp.deadline != 0 && wall_clock64() > sp.deadline) return AMX_MAX_TIME_EXCEEDED;
-      double nrm = wrms(zn[0], ewt);
-      if constexpr (QUAD) if (quad_errcon()) nrm = fmax(nrm, wrmsQ(znQ[0], ewtQ));
+      double nrm = wrms(Z(0), ewt);
+      if constexpr (QUAD) {
+        if (quad_errcon()) {
+          double z0[EPQ];
+          znQ0(z0);
+          nrm = fmax(nrm, wrmsQ(z0, ewtQ));
+        }
+      }
       if constexpr (SENS) nrm = fmax(nrm, pb.sens_norm_zn(0));
+      if (!(nrm == nrm)) return AMX_ERR_FAILURE;
       tolsf = UROUND * nrm;
       if (tolsf > 1.0) return AMX_TOO_MUCH_ACC;
       tolsf = 1.0;
@@ -985,7 +1084,7 @@ struct BDF {
       if (kflag != AMX_SUCCESS) return kflag;
       const double troundoff = FUZZ_FACTOR * UROUND * (fabs(tn) + fabs(h));
       if (fabs(tn - tstop) <= troundoff) tn = tstop;
-      if (!pb.on_step(tn, zn[0], zn[1], hscale)) return AMX_TRAJ_FULL;
+      if (!pb.on_step(tn, Z(0), Z(1), hscale)) return AMX_TRAJ_FULL;
       if (tn == tstop) {
         tstopset = false;
         return AMX_SUCCESS;

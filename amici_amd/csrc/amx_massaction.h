@@ -73,7 +73,8 @@ struct Smem {
 template <class M>
 struct Launch {
   static constexpr int BY_THREADS = 256 / M::T;
-  static constexpr int BY_LDS = (int)(65536 / sizeof(Smem<M>));
+  static constexpr int BY_LDS =
+      (int)(65536 / (sizeof(Smem<M>) + sizeof(BdfLds<M::NX, M::NP>)));
   static constexpr int TEAMS = (BY_THREADS < BY_LDS ? BY_THREADS : BY_LDS) > 0
                                    ? (BY_THREADS < BY_LDS ? BY_THREADS : BY_LDS) : 1;
   static constexpr int THREADS = TEAMS * M::T < 64 ? 64 : TEAMS * M::T;
@@ -87,15 +88,15 @@ struct MAEval {
   const Team<T>& tm;
   Smem<M>& s;
 
-  __device__ MAEval(const Team<T>& t, Smem<M>& sm) : tm(t), s(sm) {}
+  __device__ __forceinline__ MAEval(const Team<T>& t, Smem<M>& sm) : tm(t), s(sm) {}
 
-  __device__ void put_x(double x) {
+  __device__ __forceinline__ void put_x(double x) {
     team_sync();
     if (tm.lane < NX) s.x[tm.lane] = x;
     team_sync();
   }
   // monomials m_r = prod x_reactants into s.w (x already in s.x)
-  __device__ void monomials() {
+  __device__ __forceinline__ void monomials() {
     for (int r = tm.lane; r < NR; r += T) {
       const int a = M::RA[r], b = M::RB[r];
       double m = (a >= 0) ? s.x[a] : 1.0;
@@ -104,7 +105,7 @@ struct MAEval {
     }
     team_sync();
   }
-  __device__ double xdot(double x) {
+  __device__ __forceinline__ double xdot(double x) {
     put_x(x);
     for (int r = tm.lane; r < NR; r += T) {
       const int a = M::RA[r], b = M::RB[r];
@@ -121,7 +122,7 @@ struct MAEval {
     return f;
   }
   // dense J into s.J (row-major, leading dim JLD); x already in s.x
-  __device__ void jacobian() {
+  __device__ __forceinline__ void jacobian() {
     constexpr int JLD = Smem<M>::JLD;
     for (int e = tm.lane; e < NX * NX; e += T) s.J[(e / NX) * JLD + (e % NX)] = 0.0;
     team_sync();
@@ -138,7 +139,7 @@ struct MAEval {
     team_sync();
   }
   // u_r = sum_i S_ir v_i into s.u; v in s.v
-  __device__ void stoich_t() {
+  __device__ __forceinline__ void stoich_t() {
     for (int r = tm.lane; r < NR; r += T) {
       double u = 0.0;
       for (int q = M::RS_PTR[r]; q < M::RS_PTR[r + 1]; ++q)
@@ -165,11 +166,11 @@ struct FwdProblem {
   double* tf;
   int ntraj, max_traj;
 
-  __device__ FwdProblem(const Team<T>& t, Smem<M>& sm, double* w)
+  __device__ __forceinline__ FwdProblem(const Team<T>& t, Smem<M>& sm, double* w)
       : tm(t), s(sm), ev(t, sm), ws(w), tt(nullptr), ty(nullptr), tf(nullptr), ntraj(0),
         max_traj(0) {}
 
-  __device__ bool on_step(double tn, double z0, double z1, double hs) {
+  __device__ __forceinline__ bool on_step(double tn, double z0, double z1, double hs) {
     if (tt == nullptr) return true;
     if (ntraj >= max_traj) return false;
     if (tm.lane == 0) tt[ntraj] = tn;
@@ -181,13 +182,13 @@ struct FwdProblem {
     return true;
   }
 
-  __device__ double rhs(double, double y) { return ev.xdot(y); }
-  __device__ void quad_rhs(double, double, double*) {}
-  __device__ void jac(double, double y) {
+  __device__ __forceinline__ double rhs(double, double y) { return ev.xdot(y); }
+  __device__ __forceinline__ void quad_rhs(double, double, double*) {}
+  __device__ __forceinline__ void jac(double, double y) {
     ev.put_x(y);
     ev.jacobian();
   }
-  __device__ void build_M(double gamma, double* a) {
+  __device__ __forceinline__ void build_M(double gamma, double* a) {
     const int i = tm.lane;
 #pragma unroll
     for (int j = 0; j < NX; ++j)
@@ -195,7 +196,7 @@ struct FwdProblem {
   }
 
   // ---------- forward sensitivities (global-memory Nordsieck arrays) ----------
-  __device__ double& S(int slot, int is) const {
+  __device__ __forceinline__ double& S(int slot, int is) const {
     const int i = tm.lane < NX ? tm.lane : NX - 1;
     return ws[((size_t)slot * NS + is) * NX + i];
   }
@@ -203,13 +204,13 @@ struct FwdProblem {
   double ssp_rtol, ssp_atol;
 
   // Stages y (for J*v through the flux structure) and the monomials (df/dp).
-  __device__ void sens_prepare(double y) {
+  __device__ __forceinline__ void sens_prepare(double y) {
     ev.put_x(y);
     ev.monomials();
   }
   // sensitivity RHS for column is at the staged y:
   //   (J v)_i + df_i/dp_is = sum_r S_ir (dw_r/dx . v + [KI(r) == is] m_r)
-  __device__ double sens_f(double yS, int is) {
+  __device__ __forceinline__ double sens_f(double yS, int is) {
     team_sync();
     if (tm.lane < NX) s.v[tm.lane] = yS;
     team_sync();
@@ -229,7 +230,7 @@ struct FwdProblem {
         acc = fma(M::SP_C[q], s.u[M::SP_R[q]], acc);
     return acc;
   }
-  __device__ double sens_norm_slot(int slot) const {
+  __device__ __forceinline__ double sens_norm_slot(int slot) const {
     double m = 0.0;
     for (int is = 0; is < NS; ++is) {
       const double e = (tm.lane < NX) ? S(slot, is) * S(EWT, is) : 0.0;
@@ -237,16 +238,16 @@ struct FwdProblem {
     }
     return m;
   }
-  __device__ void sens_init() {
+  __device__ __forceinline__ void sens_init() {
     if (tm.lane < NX)
       for (int sl = 0; sl < SENS_SLOTS; ++sl)
         for (int is = 0; is < NS; ++is) S(sl, is) = 0.0;  // sx0 = dx0/dp = 0
   }
-  __device__ void sens_set_ewt() {
+  __device__ __forceinline__ void sens_set_ewt() {
     if (tm.lane < NX)
       for (int is = 0; is < NS; ++is) S(EWT, is) = 1.0 / (ssp_rtol * fabs(S(0, is)) + ssp_atol);
   }
-  __device__ void sens_predict(int q) {
+  __device__ __forceinline__ void sens_predict(int q) {
     if (tm.lane >= NX) return;
     for (int is = 0; is < NS; ++is) {
       double z[LMAX];
@@ -262,7 +263,7 @@ struct FwdProblem {
         if (j < q) S(j, is) = z[j];
     }
   }
-  __device__ void sens_restore(int q) {
+  __device__ __forceinline__ void sens_restore(int q) {
     if (tm.lane >= NX) return;
     for (int is = 0; is < NS; ++is) {
       double z[LMAX];
@@ -278,7 +279,7 @@ struct FwdProblem {
         if (j < q) S(j, is) = z[j];
     }
   }
-  __device__ void sens_rescale(int q, double eta) {
+  __device__ __forceinline__ void sens_rescale(int q, double eta) {
     if (tm.lane >= NX) return;
     for (int is = 0; is < NS; ++is) {
       double c = eta;
@@ -289,7 +290,7 @@ struct FwdProblem {
       }
     }
   }
-  __device__ void sens_increase(int q, double A1, const double (&ll)[LMAX]) {
+  __device__ __forceinline__ void sens_increase(int q, double A1, const double (&ll)[LMAX]) {
     if (tm.lane >= NX) return;
     for (int is = 0; is < NS; ++is) {
       const double zL = A1 * S(QMAX, is);
@@ -299,7 +300,7 @@ struct FwdProblem {
       S(q + 1, is) = zL;
     }
   }
-  __device__ void sens_decrease(int q, const double (&ll)[LMAX]) {
+  __device__ __forceinline__ void sens_decrease(int q, const double (&ll)[LMAX]) {
     if (tm.lane >= NX) return;
     for (int is = 0; is < NS; ++is) {
       const double zq = S(q, is);
@@ -308,7 +309,7 @@ struct FwdProblem {
         if (j < q) S(j, is) = fma(-ll[j], zq, S(j, is));
     }
   }
-  __device__ void sens_complete(int q, const double (&l)[LMAX]) {
+  __device__ __forceinline__ void sens_complete(int q, const double (&l)[LMAX]) {
     if (tm.lane >= NX) return;
     for (int is = 0; is < NS; ++is) {
       const double a = S(ACOR, is);
@@ -317,12 +318,12 @@ struct FwdProblem {
         if (j <= q) S(j, is) = fma(l[j], a, S(j, is));
     }
   }
-  __device__ void sens_save_acor() {
+  __device__ __forceinline__ void sens_save_acor() {
     if (tm.lane >= NX) return;
     for (int is = 0; is < NS; ++is) S(QMAX, is) = S(ACOR, is);
   }
-  __device__ double sens_norm_zn(int j) { return sens_norm_slot(j); }
-  __device__ double sens_norm_qp1(double cquot) {
+  __device__ __forceinline__ double sens_norm_zn(int j) { return sens_norm_slot(j); }
+  __device__ __forceinline__ double sens_norm_qp1(double cquot) {
     double m = 0.0;
     for (int is = 0; is < NS; ++is) {
       const double e = (tm.lane < NX) ? (S(ACOR, is) - cquot * S(QMAX, is)) * S(EWT, is) : 0.0;
@@ -330,7 +331,7 @@ struct FwdProblem {
     }
     return m;
   }
-  __device__ double sens_hub_inv() {
+  __device__ __forceinline__ double sens_hub_inv() {
     double m = 0.0;
     if (tm.lane < NX)
       for (int is = 0; is < NS; ++is)
@@ -338,24 +339,24 @@ struct FwdProblem {
     return tm.max(m);
   }
   // znS[1] = fS(t, y, znS[0]) (unscaled)
-  __device__ void sens_init_deriv(double, double y, double) {
+  __device__ __forceinline__ void sens_init_deriv(double, double y, double) {
     sens_prepare(y);
     for (int is = 0; is < NS; ++is) {
       const double v = sens_f(tm.lane < NX ? S(0, is) : 0.0, is);
       if (tm.lane < NX) S(1, is) = v;
     }
   }
-  __device__ void sens_scale_deriv(double h) {
+  __device__ __forceinline__ void sens_scale_deriv(double h) {
     if (tm.lane < NX)
       for (int is = 0; is < NS; ++is) S(1, is) *= h;
   }
-  __device__ void sens_restart_q1(double t, double y, double f, double h) {
+  __device__ __forceinline__ void sens_restart_q1(double t, double y, double f, double h) {
     sens_init_deriv(t, y, f);
     sens_scale_deriv(h);
   }
   double ydd_hg;
-  __device__ void sens_ydd_prepare(double hg) { ydd_hg = hg; }
-  __device__ double sens_ydd_norm(double, double yy, double, double hg) {
+  __device__ __forceinline__ void sens_ydd_prepare(double hg) { ydd_hg = hg; }
+  __device__ __forceinline__ double sens_ydd_norm(double, double yy, double, double hg) {
     sens_prepare(yy);
     double m = 0.0;
     for (int is = 0; is < NS; ++is) {
@@ -370,7 +371,7 @@ struct FwdProblem {
   // cvStgrNls: Newton on all sensitivity corrections with the state's
   // Newton matrix (no setup unless the iteration fails with a stale J).
   template <class B>
-  __device__ int sens_nls(B& b) {
+  __device__ __forceinline__ int sens_nls(B& b) {
     sens_prepare(b.y);
     bool callSetup = false, jbad = false;
     for (;;) {
@@ -398,7 +399,7 @@ struct FwdProblem {
           Del = fmax(Del, sqrt(tm.sum(e * e) / NX));
         }
         if (m > 0) b.crateS = fmax(CRDOWN * b.crateS, Del / b.delp);
-        const double dcon = Del * fmin(1.0, b.crateS) / b.tq[4];
+        const double dcon = Del * fmin(1.0, b.crateS) / b.S.tq[4];
         if (dcon <= 1.0) {
           b.acnrmS = (m == 0) ? Del : sens_norm_slot(ACOR);
           b.jcur = false;
@@ -438,9 +439,9 @@ struct BwdProblem {
   double ta, tb, ya, yb, fa, fb;
   double t_cached, x_cached;
 
-  __device__ bool on_step(double, double, double, double) { return true; }
+  __device__ __forceinline__ bool on_step(double, double, double, double) { return true; }
 
-  __device__ BwdProblem(const Team<T>& t, Smem<M>& sm, const double* tt_,
+  __device__ __forceinline__ BwdProblem(const Team<T>& t, Smem<M>& sm, const double* tt_,
                                         const double* ty_, const double* tf_, int n_)
       : tm(t), s(sm), ev(t, sm), tt(tt_), ty(ty_), tf(tf_), n(n_) {
     cur = -1;
@@ -448,7 +449,7 @@ struct BwdProblem {
     x_cached = 0.0;
   }
 
-  __device__ void load(int k) {
+  __device__ __forceinline__ void load(int k) {
     cur = k;
     ta = tt[k - 1];
     tb = tt[k];
@@ -459,7 +460,7 @@ struct BwdProblem {
     fb = tf[(size_t)k * NX + i];
   }
 
-  __device__ double xf(double t) {
+  __device__ __forceinline__ double xf(double t) {
     if (t == t_cached && cur > 0) return x_cached;
     if (n < 2) {
       x_cached = ty[tm.lane < NX ? tm.lane : 0];
@@ -483,7 +484,7 @@ struct BwdProblem {
   }
 
   // puts x(t) and the adjoint state into LDS and computes u = S^T xB
-  __device__ void stage(double t, double yB) {
+  __device__ __forceinline__ void stage(double t, double yB) {
     const double x = xf(t);
     team_sync();
     if (tm.lane < NX) {
@@ -494,7 +495,7 @@ struct BwdProblem {
     ev.stoich_t();
   }
 
-  __device__ double rhs(double t, double yB) {
+  __device__ __forceinline__ double rhs(double t, double yB) {
     stage(t, yB);
     double f = 0.0;
     if (tm.lane < NX)
@@ -507,7 +508,7 @@ struct BwdProblem {
     return f;
   }
 
-  __device__ void quad_rhs(double t, double yB, double* fq) {
+  __device__ __forceinline__ void quad_rhs(double t, double yB, double* fq) {
     stage(t, yB);
     ev.monomials();
 #pragma unroll
@@ -523,13 +524,13 @@ struct BwdProblem {
     }
   }
 
-  __device__ void jac(double t, double) {
+  __device__ __forceinline__ void jac(double t, double) {
     const double x = xf(t);
     ev.put_x(x);
     ev.jacobian();
   }
   // Newton matrix of the backward problem: I - gamma * (-J^T)
-  __device__ void build_M(double gamma, double* a) {
+  __device__ __forceinline__ void build_M(double gamma, double* a) {
     const int i = tm.lane;
 #pragma unroll
     for (int j = 0; j < NX; ++j)
@@ -545,7 +546,7 @@ struct ObjOut {
   double dJdx;  // dJy/dx of this lane's state
 };
 template <class M>
-__device__ ObjOut objective_at(const Team<M::T>& tm, Smem<M>& s, double x,
+__device__ __forceinline__ ObjOut objective_at(const Team<M::T>& tm, Smem<M>& s, double x,
                                                const double* meas, const double* sigma,
                                                double* yout) {
   constexpr int NY = M::NY, NX = M::NX, T = M::T;
@@ -582,7 +583,7 @@ __device__ ObjOut objective_at(const Team<M::T>& tm, Smem<M>& s, double x,
   return ObjOut{tm.sum(nllh), d};
 }
 
-__device__ bool next_t_too_close(double cur, double next) {
+__device__ __forceinline__ bool next_t_too_close(double cur, double next) {
   const double tdiff = next - cur;
   if (tdiff == 0.0) return true;
   const double tround = UROUND * fmax(fabs(cur), fabs(next));
@@ -590,7 +591,7 @@ __device__ bool next_t_too_close(double cur, double next) {
 }
 
 template <class B>
-__device__ void store_stats(int* st, const B& b, int off) {
+__device__ __forceinline__ void store_stats(int* st, const B& b, int off) {
   st[off + 0] = b.nst;
   st[off + 1] = b.nfe;
   st[off + 2] = b.nsetups;
@@ -605,7 +606,8 @@ __device__ void store_stats(int* st, const B& b, int off) {
 // forward simulation of one instance.  MODE 0: states, 1: FSA, 2: adjoint
 // forward pass (stores the trajectory and dJy/dx at every output).
 template <class M, int MODE>
-__device__ void simulate_forward(int inst, const BatchArgs& a, const Team<M::T>& tm, Smem<M>& s) {
+__device__ __forceinline__ void simulate_forward(int inst, const BatchArgs& a, const Team<M::T>& tm,
+                                                 Smem<M>& s, BdfLds<M::NX, 0>& lds) {
   constexpr int NX = M::NX, NP = M::NP, NK = M::NK, NY = M::NY, T = M::T;
   constexpr int NS = (MODE == 1) ? NP : 0;
   const size_t I = (size_t)inst;
@@ -628,7 +630,7 @@ __device__ void simulate_forward(int inst, const BatchArgs& a, const Team<M::T>&
     pb0.max_traj = a.max_traj;
   }
   PB& pb = pb0;
-  BDF<PB> bdf(pb, a.sp);
+  BDF<PB> bdf(pb, lds, a.sp);
   if (a.maxtime_ticks) bdf.sp.deadline = wall_clock64() + a.maxtime_ticks;
   bdf.init(0.0, x0, nullptr);
   if (a.trace && inst == a.trace_inst) {
@@ -647,7 +649,7 @@ __device__ void simulate_forward(int inst, const BatchArgs& a, const Team<M::T>&
       status = bdf.advance(tout);
       if (status != AMX_SUCCESS) break;
     }
-    const double x = bdf.zn[0];
+    const double x = bdf.Z(0);
     if (a.x && tm.lane < NX) a.x[(I * a.nt + it) * NX + tm.lane] = x;
     const size_t mo = (I * a.nt + it) * NY;
     const ObjOut ob = objective_at<M>(tm, s, x, a.meas ? a.meas + mo : nullptr, a.sigma + mo,
@@ -700,7 +702,8 @@ __device__ void simulate_forward(int inst, const BatchArgs& a, const Team<M::T>&
 // (src/backwardproblem.cpp:267-332) with a fresh order-1 restart after every
 // data-point jump (CVodeSolver::reinit_b, src/solver_cvodes.cpp:594-649).
 template <class M>
-__device__ void simulate_backward(int inst, const BatchArgs& a, const Team<M::T>& tm, Smem<M>& s) {
+__device__ __forceinline__ void simulate_backward(int inst, const BatchArgs& a, const Team<M::T>& tm,
+                                                  Smem<M>& s, BdfLds<M::NX, M::NP>& lds) {
   constexpr int NX = M::NX, NP = M::NP, T = M::T;
   using PB = BwdProblem<M>;
   constexpr int EPQ = PB::EPQ;
@@ -716,7 +719,7 @@ __device__ void simulate_backward(int inst, const BatchArgs& a, const Team<M::T>
   const double* tt = a.traj_t + I * (size_t)a.max_traj;
   PB pb0(tm, s, tt, a.traj_y + I * (size_t)a.max_traj * NX,
          a.traj_f + I * (size_t)a.max_traj * NX, n);
-  BDF<PB> bdf(pb0, a.spB);  // holds pb0 by reference
+  BDF<PB> bdf(pb0, lds, a.spB);  // holds pb0 and the LDS block by reference
   if (a.maxtime_ticks) bdf.sp.deadline = wall_clock64() + a.maxtime_ticks;
   const double* ts = a.ts + (size_t)a.ts_stride * I;
   const double* dJ = a.dJydx + I * a.nt * NX;
@@ -745,9 +748,9 @@ __device__ void simulate_backward(int inst, const BatchArgs& a, const Team<M::T>
       status = bdf.advance(tnext);
       if (status != AMX_SUCCESS) break;
       tcur = tnext;
-      xB = bdf.zn[0];
+      xB = bdf.Z(0);
 #pragma unroll
-      for (int e = 0; e < EPQ; ++e) qB[e] = bdf.znQ[0][e];
+      for (int e = 0; e < EPQ; ++e) qB[e] = bdf.ZQ(0, e);
     }
     if (tm.lane < NX) xB += dJ[(size_t)it * NX + tm.lane];
     --it;
@@ -758,7 +761,7 @@ __device__ void simulate_backward(int inst, const BatchArgs& a, const Team<M::T>
     status = bdf.advance(t0);
     if (status == AMX_SUCCESS) {
 #pragma unroll
-      for (int e = 0; e < EPQ; ++e) qB[e] = bdf.znQ[0][e];
+      for (int e = 0; e < EPQ; ++e) qB[e] = bdf.ZQ(0, e);
     }
   }
   AMX_ACC_STATS();
@@ -792,28 +795,32 @@ __device__ __forceinline__ int next_instance(const Team<M::T>& tm, int* counter)
 }
 
 template <class M, int MODE>
-__device__ void forward_kernel_body(BatchArgs a) {
+__device__ __forceinline__ void forward_kernel_body(BatchArgs a) {
   constexpr int T = M::T;
   __shared__ Smem<M> smem[Launch<M>::TEAMS];
+  __shared__ BdfLds<M::NX, 0> blds[Launch<M>::TEAMS];
   Team<T> tm;
   Smem<M>& s = smem[threadIdx.x / T];
-  for (;;) {
+  BdfLds<M::NX, 0>& lds = blds[threadIdx.x / T];
+  for (int guard = 0; guard <= a.nb; ++guard) {
     const int inst = next_instance<M>(tm, a.counter);
     if (inst >= a.nb) break;
-    simulate_forward<M, MODE>(inst, a, tm, s);
+    simulate_forward<M, MODE>(inst, a, tm, s, lds);
   }
 }
 
 template <class M>
-__device__ void backward_kernel_body(BatchArgs a) {
+__device__ __forceinline__ void backward_kernel_body(BatchArgs a) {
   constexpr int T = M::T;
   __shared__ Smem<M> smem[Launch<M>::TEAMS];
+  __shared__ BdfLds<M::NX, M::NP> blds[Launch<M>::TEAMS];
   Team<T> tm;
   Smem<M>& s = smem[threadIdx.x / T];
-  for (;;) {
+  BdfLds<M::NX, M::NP>& lds = blds[threadIdx.x / T];
+  for (int guard = 0; guard <= a.nb; ++guard) {
     const int inst = next_instance<M>(tm, a.counter);
     if (inst >= a.nb) break;
-    simulate_backward<M>(inst, a, tm, s);
+    simulate_backward<M>(inst, a, tm, s, lds);
   }
 }
 

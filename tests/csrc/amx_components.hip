@@ -35,13 +35,16 @@ __global__ void __launch_bounds__(THREADS) k_lu(const double* A, const double* b
   amx::Team<T> tm;
   const int team = (blockIdx.x * blockDim.x + threadIdx.x) / T;
   if (team >= nteams) return;
+  __shared__ amx::BdfLds<NX, 0> lds[TEAMS];
   amx::LURows<NX, T> lu;
   const int i = tm.lane;
+  double a[NX];
 #pragma unroll
-  for (int j = 0; j < NX; ++j) lu.a[j] = (i < NX) ? A[((size_t)team * NX + i) * NX + j] : 0.0;
-  const bool good = lu.factor(tm);
+  for (int j = 0; j < NX; ++j) a[j] = (i < NX) ? A[((size_t)team * NX + i) * NX + j] : 0.0;
+  amx::BdfLds<NX, 0>& S = lds[threadIdx.x / T];
+  const bool good = lu.factor(tm, a, S);
   const double bi = (i < NX) ? b[(size_t)team * NX + i] : 0.0;
-  const double xi = lu.solve(bi, tm);
+  const double xi = lu.solve(bi, tm, S);
   if (i < NX) x[(size_t)team * NX + i] = good ? xi : NAN;
   if (i == 0) ok[team] = good ? 1 : 0;
 }
