@@ -93,9 +93,12 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--plain-sample", type=int, default=8,
-                    help="conditions timed on the plain GPU BDF path")
+                    help="conditions timed on the plain GPU BDF path (0 = skip)")
     ap.add_argument("--cpu-sample", type=int, default=128,
                     help="conditions timed on the CPU baseline (0 = skip)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a device-to-device copy of BYTES bytes 5 times (the known-size "
+                         "kernel that scales the HBM counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the batched path shards by instances across GPUs "
@@ -112,6 +115,15 @@ def main() -> int:
         return 1
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.copy_calibration:
+        n = args.copy_calibration // 8
+        src = torch.rand(n, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n * 8, "launches": 5}))
+        return 0
     cs = networks.condition_set("N40")
     net = cs["network"]
     model = sim.Model(net)
@@ -186,10 +198,46 @@ def main() -> int:
                 "whole_step_tflops": float((f_fwd + f_bwd).sum()) / (dt / args.steps) / 1e12}
 
     # plain GPU BDF: one instance per launch, forward sensitivities
+    plain, grad_rel = None, None
+    if args.plain_sample > 0:
+        plain, grad_rel = plain_gpu_bdf(model, theta, kfix, ts, meas, sigma, sllh, nb, args.plain_sample)
+
+    cpu = cpu_baseline(cs, cs["theta"], cs["kfix"], meas_np, sigma_np, args.cpu_sample) \
+        if args.cpu_sample > 0 else None
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "N40 adjoint NLL+gradient: 40 species, 120 reactions, "
+                               "1024 input conditions, 25 times, 30 log-normal observables",
+                   "instances": nb, "np": net.np, "rtol": TOL["rtol"], "atol": TOL["atol"],
+                   "sensitivity": "adjoint"},
+        "instances_ok": n_ok,
+        "plain_gpu_bdf": plain,
+        "speedup_vs_plain_gpu_bdf": (value / plain["value"]) if plain else None,
+        "sllh_adjoint_vs_forward_rel": grad_rel,
+        "solver_steps_mean": {"fwd": float(stats[:, 0].mean()), "bwd": float(stats[:, 8].mean()),
+                              "cvodes_fwd": float(gold[:, 0].mean()),
+                              "cvodes_bwd": float(gold[:, 9].mean())},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def plain_gpu_bdf(model, theta, kfix, ts, meas, sigma, sllh, nb, S):
+    """The same integrator, one instance per kernel launch, forward
+    sensitivities, on the first S conditions; also returns the largest
+    relative difference between its sllh and the batched adjoint's."""
+    import torch
+
+    from amici_amd import sim
+
     fsa = sim.Solver(**TOL)
     fsa.set_sensitivity_order(sim.SensitivityOrder.first)
     fsa.set_sensitivity_method(sim.SensitivityMethod.forward)
-    S = args.plain_sample
     sl = slice(0, S)
     sim.simulate_batch(model, fsa, theta[:1], kfix[:1], ts, meas[:1], sigma[:1], sequential=True,
                        want_x=False, want_y=False)
@@ -206,30 +254,7 @@ def main() -> int:
     a_s = sllh[:S]
     f_s = pl["sllh"].cpu().numpy()
     grad_rel = float(np.max(np.abs(a_s - f_s) / (np.abs(f_s) + 1e-7 * np.abs(f_s).max())))
-
-    cpu = cpu_baseline(cs, cs["theta"], cs["kfix"], meas_np, sigma_np, args.cpu_sample) \
-        if args.cpu_sample > 0 else None
-
-    line = {
-        "metric": METRIC, "value": value, "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "N40 adjoint NLL+gradient: 40 species, 120 reactions, "
-                               "1024 input conditions, 25 times, 30 log-normal observables",
-                   "instances": nb, "np": net.np, "rtol": TOL["rtol"], "atol": TOL["atol"],
-                   "sensitivity": "adjoint"},
-        "instances_ok": n_ok,
-        "plain_gpu_bdf": plain,
-        "speedup_vs_plain_gpu_bdf": value / plain["value"],
-        "sllh_adjoint_vs_forward_rel": grad_rel,
-        "solver_steps_mean": {"fwd": float(stats[:, 0].mean()), "bwd": float(stats[:, 8].mean()),
-                              "cvodes_fwd": float(gold[:, 0].mean()),
-                              "cvodes_bwd": float(gold[:, 9].mean())},
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(line), flush=True)
-    return 0
+    return plain, grad_rel
 
 
 if __name__ == "__main__":
