@@ -147,20 +147,29 @@ struct LURows {
   }
 
   // Solves (LU) x = b; b is this lane's component of the right-hand side.
+  // The lane's row of factors and the pivot order are loaded from LDS once,
+  // before the substitution loops, so the loops run on registers.
   template <class L>
   __device__ __forceinline__ double solve(double b, const Team<T>& tm, const L& S) const {
     const bool act = tm.lane < NX;
     const int cl = act ? tm.lane : NX - 1;
+    double r[NX];
+    int pv[NX];
 #pragma unroll
     for (int k = 0; k < NX; ++k) {
-      const double yk = tm.bcast(b, S.perm[k]);
-      if (act && pos > k) b = fma(-S.lu[k][cl], yk, b);
+      r[k] = S.lu[k][cl];
+      pv[k] = __builtin_amdgcn_readfirstlane(S.perm[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < NX; ++k) {
+      const double yk = tm.bcast(b, pv[k]);
+      if (act && pos > k) b = fma(-r[k], yk, b);
     }
 #pragma unroll
     for (int k = NX - 1; k >= 0; --k) {
-      const double xk = tm.bcast(b * dinv, S.perm[k]);
+      const double xk = tm.bcast(b * dinv, pv[k]);
       if (pos == k) b = xk;
-      else if (act && pos < k) b = fma(-S.lu[k][cl], xk, b);
+      else if (act && pos < k) b = fma(-r[k], xk, b);
     }
     return tm.gather(b, src);
   }

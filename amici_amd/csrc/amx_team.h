@@ -9,12 +9,13 @@
 namespace amx {
 
 __device__ __forceinline__ void team_sync() {
-  // Orders this wavefront's LDS traffic (s_waitcnt) and stops the compiler
-  // from moving LDS accesses across the point; LDS requests of one wavefront
-  // are serviced in issue order, so this is sufficient inside a wavefront.
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  // A team never spans more than one wavefront, so wavefront-scope fences
+  // order its LDS traffic: they keep the compiler from moving LDS accesses
+  // across the point, and LDS requests of one wavefront are serviced in issue
+  // order.  (Workgroup scope would also wait for outstanding global stores.)
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Value of v on wave lane `lane` (lane uniform).  The double is split into
@@ -37,15 +38,19 @@ __device__ __forceinline__ double split_d(double v, F&& f) {
   return __builtin_bit_cast(double, r);
 }
 
-// Exchange with lane (lane ^ O) inside each group of 32 lanes: ds_swizzle in
-// bit-mask mode (and 0x1f, or 0, xor O), an immediate pattern.
-template <int O>
-__device__ __forceinline__ double swz_xor(double v) {
-  return split_d(v, [](int x) { return __builtin_amdgcn_ds_swizzle(x, (O << 10) | 0x1F); });
+// Value of lane partner(lane) inside each row of 16 lanes, through a DPP move
+// (VALU, no LDS round trip).  Controls used: quad_perm [1,0,3,2] (lane ^ 1),
+// quad_perm [2,3,0,1] (lane ^ 2), row_half_mirror (7 - lane within 8),
+// row_mirror (15 - lane within 16).
+constexpr int DPP_XOR1 = 0xB1, DPP_XOR2 = 0x4E, DPP_HALF_MIRROR = 0x141, DPP_MIRROR = 0x140;
+
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int x) {
+  return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xF, 0xF, false);
 }
-template <int O>
-__device__ __forceinline__ int swz_xor_i(int x) {
-  return __builtin_amdgcn_ds_swizzle(x, (O << 10) | 0x1F);
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  return split_d(v, [](int x) { return dpp_i<CTRL>(x); });
 }
 
 template <int T>
@@ -60,25 +65,25 @@ struct Team {
     base = wl - lane;
   }
 
-  // Reductions.  For a full-wave team (T = 64) each 32-lane half reduces
-  // with immediate-pattern swizzles and the two half results are combined
-  // from lanes 0 and 32 with readlane, so the result is one scalar value for
-  // the whole wave (uniform by construction).  Smaller teams use a butterfly
-  // of lane shuffles; every lane of the team then ends with the bitwise-same
-  // value because each level combines the same pair of operands.
+  // Reductions.  For a full-wave team (T = 64) each row of 16 lanes reduces
+  // with DPP moves (xor 1, xor 2, half mirror, mirror: every lane of the row
+  // combines the same operand pairs, so the row result is the same on all its
+  // lanes) and the four row results are combined from lanes 0, 16, 32 and 48
+  // with readlane in a fixed order, so the result is one scalar value for the
+  // whole wave (uniform by construction).  Smaller teams use a butterfly of
+  // lane shuffles.
   template <class Op>
-  __device__ __forceinline__ double half_reduce(double v, Op op) const {
-    v = op(v, swz_xor<16>(v));
-    v = op(v, swz_xor<8>(v));
-    v = op(v, swz_xor<4>(v));
-    v = op(v, swz_xor<2>(v));
-    v = op(v, swz_xor<1>(v));
+  __device__ __forceinline__ double row_reduce(double v, Op op) const {
+    v = op(v, dpp_d<DPP_XOR1>(v));
+    v = op(v, dpp_d<DPP_XOR2>(v));
+    v = op(v, dpp_d<DPP_HALF_MIRROR>(v));
+    v = op(v, dpp_d<DPP_MIRROR>(v));
     return v;
   }
   __device__ __forceinline__ double sum(double v) const {
     if constexpr (T == 64) {
-      v = half_reduce(v, [](double a, double b) { return a + b; });
-      return readlane_d(v, 0) + readlane_d(v, 32);
+      v = row_reduce(v, [](double a, double b) { return a + b; });
+      return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
     } else {
 #pragma unroll
       for (int o = T / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, T);
@@ -87,8 +92,9 @@ struct Team {
   }
   __device__ __forceinline__ double max(double v) const {
     if constexpr (T == 64) {
-      v = half_reduce(v, [](double a, double b) { return fmax(a, b); });
-      return fmax(readlane_d(v, 0), readlane_d(v, 32));
+      v = row_reduce(v, [](double a, double b) { return fmax(a, b); });
+      return fmax(fmax(readlane_d(v, 0), readlane_d(v, 16)),
+                  fmax(readlane_d(v, 32), readlane_d(v, 48)));
     } else {
 #pragma unroll
       for (int o = T / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, T));
@@ -98,13 +104,15 @@ struct Team {
   __device__ __forceinline__ int maxi(int v) const {
     if constexpr (T == 64) {
       int u;
-      u = swz_xor_i<16>(v); v = u > v ? u : v;
-      u = swz_xor_i<8>(v); v = u > v ? u : v;
-      u = swz_xor_i<4>(v); v = u > v ? u : v;
-      u = swz_xor_i<2>(v); v = u > v ? u : v;
-      u = swz_xor_i<1>(v); v = u > v ? u : v;
-      const int a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 32);
-      return a > b ? a : b;
+      u = dpp_i<DPP_XOR1>(v); v = u > v ? u : v;
+      u = dpp_i<DPP_XOR2>(v); v = u > v ? u : v;
+      u = dpp_i<DPP_HALF_MIRROR>(v); v = u > v ? u : v;
+      u = dpp_i<DPP_MIRROR>(v); v = u > v ? u : v;
+      int r = __builtin_amdgcn_readlane(v, 0);
+      u = __builtin_amdgcn_readlane(v, 16); r = u > r ? u : r;
+      u = __builtin_amdgcn_readlane(v, 32); r = u > r ? u : r;
+      u = __builtin_amdgcn_readlane(v, 48); r = u > r ? u : r;
+      return r;
     } else {
 #pragma unroll
       for (int o = T / 2; o > 0; o >>= 1) {
@@ -118,23 +126,27 @@ struct Team {
   __device__ __forceinline__ double argmax(double v, int& idx) const {
     int i = lane;
     if constexpr (T == 64) {
-#define AMX_ARGMAX_LEVEL(O)                                  \
+#define AMX_ARGMAX_STEP(CTRL)                                \
       {                                                      \
-        const double u = swz_xor<O>(v);                      \
-        const int j = swz_xor_i<O>(i);                       \
+        const double u = dpp_d<CTRL>(v);                     \
+        const int j = dpp_i<CTRL>(i);                        \
         if (u > v || (u == v && j < i)) { v = u; i = j; }    \
       }
-      AMX_ARGMAX_LEVEL(16)
-      AMX_ARGMAX_LEVEL(8)
-      AMX_ARGMAX_LEVEL(4)
-      AMX_ARGMAX_LEVEL(2)
-      AMX_ARGMAX_LEVEL(1)
-#undef AMX_ARGMAX_LEVEL
-      const double v0 = readlane_d(v, 0), v1 = readlane_d(v, 32);
-      const int i0 = __builtin_amdgcn_readlane(i, 0), i1 = __builtin_amdgcn_readlane(i, 32);
-      const bool hi = (v1 > v0) || (v1 == v0 && i1 < i0);
-      idx = hi ? i1 : i0;
-      return hi ? v1 : v0;
+      AMX_ARGMAX_STEP(DPP_XOR1)
+      AMX_ARGMAX_STEP(DPP_XOR2)
+      AMX_ARGMAX_STEP(DPP_HALF_MIRROR)
+      AMX_ARGMAX_STEP(DPP_MIRROR)
+#undef AMX_ARGMAX_STEP
+      double bv = readlane_d(v, 0);
+      int bi = __builtin_amdgcn_readlane(i, 0);
+#pragma unroll
+      for (int row = 16; row < 64; row += 16) {
+        const double rv = readlane_d(v, row);
+        const int ri = __builtin_amdgcn_readlane(i, row);
+        if (rv > bv || (rv == bv && ri < bi)) { bv = rv; bi = ri; }
+      }
+      idx = bi;
+      return bv;
     } else {
 #pragma unroll
       for (int o = T / 2; o > 0; o >>= 1) {
