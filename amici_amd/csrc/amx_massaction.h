@@ -449,15 +449,24 @@ struct BwdProblem {
     x_cached = 0.0;
   }
 
-  __device__ __forceinline__ void load(int k) {
-    cur = k;
-    ta = tt[k - 1];
-    tb = tt[k];
+  __device__ __forceinline__ void fetch(int k, double& t_, double& y_, double& f_) const {
     const int i = tm.lane < NX ? tm.lane : 0;
-    ya = ty[(size_t)(k - 1) * NX + i];
-    yb = ty[(size_t)k * NX + i];
-    fa = tf[(size_t)(k - 1) * NX + i];
-    fb = tf[(size_t)k * NX + i];
+    t_ = tt[k];
+    y_ = ty[(size_t)k * NX + i];
+    f_ = tf[(size_t)k * NX + i];
+  }
+  // Selects the interval [tt[k-1], tt[k]]; a move to the previous interval
+  // keeps the shared end point and loads only the new start point.
+  __device__ __forceinline__ void set_interval(int k) {
+    if (cur == k + 1) {
+      tb = ta;
+      yb = ya;
+      fb = fa;
+    } else {
+      fetch(k, tb, yb, fb);
+    }
+    fetch(k - 1, ta, ya, fa);
+    cur = k;
   }
 
   __device__ __forceinline__ double xf(double t) {
@@ -467,10 +476,11 @@ struct BwdProblem {
       t_cached = t;
       return x_cached;
     }
-    int k = (cur > 0) ? cur : n - 1;
-    while (k > 1 && t < tt[k - 1]) --k;
-    while (k < n - 1 && t > tt[k]) ++k;
-    if (k != cur) load(k);
+    if (cur <= 0) set_interval(n - 1);
+    // the backward sweep moves down through the stored steps; after a failed
+    // step it may move up again (bounded walks: at most n intervals)
+    for (int g = 0; g < n && cur > 1 && t < ta; ++g) set_interval(cur - 1);
+    for (int g = 0; g < n && cur < n - 1 && t > tb; ++g) set_interval(cur + 1);
     const double hk = tb - ta;
     const double sx = (t - ta) / hk;
     const double om = 1.0 - sx;
