@@ -134,3 +134,18 @@ def test_adjoint_gradient(models, model, fixture):
     _close(out["sllh"], g["sllh_asa"], 1e-5, 1e-7, "sllh (adjoint)")
     # the oracle's own forward and adjoint gradients agree to the same bar
     _close(g["sllh_asa"], g["sllh_fsa"], 1e-5, 1e-7, "oracle sllh asa vs fsa")
+
+
+def test_robertson_from_sbml(gpu):
+    """Robertson written as SBML (amici_amd/data/robertson.xml), read by the
+    SBML-subset reader, compiled and simulated, against the oracle's values."""
+    from amici_amd import sbml, sim
+
+    net = sbml.read_sbml(GOLD.parent.parent / "amici_amd" / "data" / "robertson.xml",
+                         observables={"yA": "A", "yC": "C"})
+    model = sim.Model(net)
+    g = _load("robertson_nominal")
+    out = _run({"m": model}, "m", g, sim.SensitivityMethod.forward)
+    assert (out["status"] == 0).all(), out["status"]
+    _close(out["x"], g["x"], Y_RTOL, Y_FLOOR, "x")
+    _close(out["sllh"], g["sllh_fsa"], 1e-5, 1e-7, "sllh (forward)")
