@@ -9,7 +9,7 @@ import ctypes
 import math
 from pathlib import Path
 
-AMX_ABI_VERSION = 1
+AMX_ABI_VERSION = 2
 AMX_NSTATS = 16
 SENSI_NONE, SENSI_FORWARD, SENSI_ADJOINT = 0, 1, 2
 
@@ -26,6 +26,8 @@ class SolverOpts(ctypes.Structure):
         ("maxsteps", ctypes.c_int), ("maxstepsB", ctypes.c_int),
         ("sensi_meth", ctypes.c_int), ("max_traj", ctypes.c_int),
         ("maxtime", ctypes.c_double),
+        ("newton_maxsteps", ctypes.c_int),
+        ("ss_rtol", ctypes.c_double), ("ss_atol", ctypes.c_double),
     ]
 
 
@@ -34,6 +36,7 @@ class BatchIn(ctypes.Structure):
         ("nb", ctypes.c_int), ("nt", ctypes.c_int), ("ts_stride", ctypes.c_int),
         ("theta", ctypes.c_void_p), ("kfix", ctypes.c_void_p), ("ts", ctypes.c_void_p),
         ("meas", ctypes.c_void_p), ("sigma", ctypes.c_void_p),
+        ("kfix_preeq", ctypes.c_void_p),
     ]
 
 
@@ -107,7 +110,8 @@ class ModelLib:
 
 def make_opts(rtol=1e-8, atol=1e-16, sensi_meth=SENSI_NONE, maxsteps=10000, rtol_fsa=math.nan,
               atol_fsa=math.nan, rtolB=math.nan, atolB=math.nan, quad_rtol=1e-8,
-              quad_atol=1e-12, maxstepsB=0, max_traj=0, maxtime=0.0) -> SolverOpts:
+              quad_atol=1e-12, maxstepsB=0, max_traj=0, maxtime=0.0, newton_maxsteps=40,
+              ss_rtol=math.nan, ss_atol=math.nan) -> SolverOpts:
     return SolverOpts(rtol, atol, rtol_fsa, atol_fsa, rtolB, atolB, quad_rtol, quad_atol,
                       int(maxsteps), int(maxstepsB), int(sensi_meth), int(max_traj),
-                      float(maxtime))
+                      float(maxtime), int(newton_maxsteps), float(ss_rtol), float(ss_atol))

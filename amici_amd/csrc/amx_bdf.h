@@ -61,6 +61,9 @@ enum : int {
   AMX_ILL_INPUT = -22,
   AMX_TOO_CLOSE = -27,
   AMX_MAX_TIME_EXCEEDED = -1000,
+  AMX_NO_STEADY_STATE = -81,
+  AMX_DAMPING_FACTOR_ERROR = -86,
+  AMX_SINGULAR_JACOBIAN = -9987,
   AMX_NOT_RUN = -1001,
   AMX_TRAJ_FULL = -1010,  // forward-trajectory buffer of an instance exhausted
 };

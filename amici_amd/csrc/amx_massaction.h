@@ -44,6 +44,10 @@ struct BatchArgs {
   double* dJydx;           // [nb][nt][NX]
   int max_traj;
   double* sens_ws;         // FSA workspace [nb][SENS_SLOTS][NP][NX]
+  const double* kfix_pre;  // [nb][NK] pre-equilibration fixed parameters or null
+  int newton_maxsteps;     // damped Newton iterations of the pre-equilibration
+  double ss_rtol, ss_atol; // steady-state tolerances
+  double* xss;             // adjoint: [nb][NX] pre-equilibrated steady state
   int* counter;            // work-queue counter (zero before launch)
   unsigned long long maxtime_ticks;  // per-instance wall-clock limit (0 = none)
   double* trace;           // optional step trace of one instance [cap][TRACE_REC]
@@ -613,6 +617,102 @@ __device__ __forceinline__ void store_stats(int* st, const B& b, int off) {
 }
 
 // ---------------------------------------------------------------------------
+// pre-equilibration to steady state (SteadyStateProblem,
+// src/forwardproblem.cpp:755-1223; NewtonsMethod::run,
+// src/steadystateproblem.cpp:74-161)
+
+// WRMS of xdot with the steady-state tolerances (WRMSComputer,
+// src/steadystateproblem.cpp:42-57); convergence when < 1.
+template <class M>
+__device__ __forceinline__ double ss_wrms(const Team<M::T>& tm, double f, double x,
+                                          const BatchArgs& a) {
+  const double e = (tm.lane < M::NX) ? f / (a.ss_rtol * fabs(x) + a.ss_atol) : 0.0;
+  return sqrt(tm.sum(e * e) / M::NX);
+}
+
+// Factorises J(x) (rows of J^T when TRANS) into the LU of `lds`.
+template <class M, bool TRANS, class L>
+__device__ __forceinline__ bool factor_jacobian(const Team<M::T>& tm, Smem<M>& s, MAEval<M>& ev,
+                                                double x, LURows<M::NX, M::T>& lu, L& lds) {
+  constexpr int NX = M::NX, JLD = Smem<M>::JLD;
+  ev.put_x(x);
+  ev.jacobian();
+  const bool act = tm.lane < NX;
+  const int cl = act ? tm.lane : 0;
+  double row[NX];
+#pragma unroll
+  for (int j = 0; j < NX; ++j)
+    row[j] = act ? (TRANS ? s.J[j * JLD + cl] : s.J[cl * JLD + j]) : 0.0;
+  return lu.factor(tm, row, lds);
+}
+
+// Damped Newton on xdot(x) = 0 from x: the direction solves J delta = -xdot
+// (NewtonSolver::get_step), the damping factor doubles after a step that
+// lowers the residual and is quartered otherwise, failing below 1e-8
+// (NewtonsMethod::update_damping_factor).  If Newton does not converge
+// within newton_maxsteps, the initial state is integrated over horizons
+// 1, 10, 100, ... until the criterion holds.  On success x is the steady
+// state; otherwise an AMICI status is returned.
+template <class M>
+__device__ __forceinline__ int steady_state(const BatchArgs& a, const Team<M::T>& tm, Smem<M>& s,
+                                            BdfLds<M::NX, 0>& lds, double& x) {
+  constexpr int NX = M::NX, T = M::T;
+  MAEval<M> ev(tm, s);
+  const bool act = tm.lane < NX;
+  const double x0 = x;
+  double f = ev.xdot(x);
+  double best = ss_wrms<M>(tm, f, x, a);
+  if (best < 1.0) return AMX_SUCCESS;
+  int status = AMX_NO_STEADY_STATE;
+  {
+    LURows<NX, T> lu;
+    double xold = x, delta = 0.0, gam = 1.0;
+    bool ok = true;
+    for (int it = 0; it < a.newton_maxsteps; ++it) {
+      if (ok) {
+        xold = x;
+        if (!factor_jacobian<M, false>(tm, s, ev, x, lu, lds)) {
+          status = AMX_SINGULAR_JACOBIAN;
+          break;
+        }
+        delta = lu.solve(act ? -f : 0.0, tm, lds);
+        if (!act) delta = 0.0;
+      }
+      x = xold + gam * delta;
+      f = ev.xdot(x);
+      const double w = ss_wrms<M>(tm, f, x, a);
+      ok = w < best;
+      if (ok) {
+        best = w;
+        if (w < 1.0) return AMX_SUCCESS;
+      }
+      gam = ok ? fmin(1.0, 2.0 * gam) : 0.25 * gam;
+      if (gam < 1e-8) {
+        status = AMX_DAMPING_FACTOR_ERROR;
+        break;
+      }
+    }
+  }
+  // integration from the initial state
+  FwdProblem<M, 0> pp(tm, s, nullptr);
+  BDF<FwdProblem<M, 0>> bdf(pp, lds, a.sp);
+  if (a.maxtime_ticks) bdf.sp.deadline = wall_clock64() + a.maxtime_ticks;
+  bdf.init(0.0, x0, nullptr);
+  double tout = 1.0;
+  for (int k = 0; k < 16; ++k, tout *= 10.0) {
+    const int st = bdf.advance(tout);
+    if (st != AMX_SUCCESS) return status == AMX_NO_STEADY_STATE ? st : status;
+    const double xv = bdf.Z(0);
+    const double fv = ev.xdot(xv);
+    if (ss_wrms<M>(tm, fv, xv, a) < 1.0) {
+      x = xv;
+      return AMX_SUCCESS;
+    }
+  }
+  return AMX_NO_STEADY_STATE;
+}
+
+// ---------------------------------------------------------------------------
 // forward simulation of one instance.  MODE 0: states, 1: FSA, 2: adjoint
 // forward pass (stores the trajectory and dJy/dx at every output).
 template <class M, int MODE>
@@ -627,6 +727,24 @@ __device__ __forceinline__ void simulate_forward(int inst, const BatchArgs& a, c
   if (tm.lane < NX) {
     const int kx = M::X0K[tm.lane];
     x0 = (kx >= 0) ? a.kfix[I * NK + kx] : M::X0C[tm.lane];
+  }
+  // pre-equilibration: steady state at the pre-equilibration inputs, then the
+  // fixed-parameter-dependent initial states are reset from kfix
+  // (reinitialize_fixed_parameter_initial_states)
+  int pre_status = AMX_SUCCESS;
+  double xss = 0.0;
+  if (a.kfix_pre) {
+    double xp = 0.0;
+    if (tm.lane < NX) {
+      const int kx = M::X0K[tm.lane];
+      xp = (kx >= 0) ? a.kfix_pre[I * NK + kx] : M::X0C[tm.lane];
+    }
+    pre_status = steady_state<M>(a, tm, s, lds, xp);
+    xss = xp;
+    if (tm.lane < NX && M::X0K[tm.lane] < 0) x0 = xp;
+    if constexpr (MODE == 2) {
+      if (tm.lane < NX) a.xss[I * NX + tm.lane] = xp;
+    }
   }
   double* ws = (MODE == 1) ? a.sens_ws + I * (size_t)SENS_SLOTS * NP * NX : nullptr;
   using PB = FwdProblem<M, NS>;
@@ -650,10 +768,27 @@ __device__ __forceinline__ void simulate_forward(int inst, const BatchArgs& a, c
   if constexpr (MODE == 1) pb.sens_init();
   const double* ts = a.ts + (size_t)a.ts_stride * I;
 
-  int status = AMX_SUCCESS;
+  int status = pre_status;
+  if constexpr (MODE == 1) {
+    // steady-state sensitivities by the implicit function theorem:
+    // sx0 = -J(x_ss)^{-1} df/dp for the species not reset from kfix
+    if (a.kfix_pre && status == AMX_SUCCESS) {
+      LURows<NX, T> lu;
+      if (!factor_jacobian<M, false>(tm, s, pb.ev, xss, lu, lds)) {
+        status = AMX_SINGULAR_JACOBIAN;
+      } else {
+        pb.sens_prepare(xss);
+        for (int is = 0; is < NP; ++is) {
+          const double fp = pb.sens_f(0.0, is);
+          const double v = lu.solve(tm.lane < NX ? -fp : 0.0, tm, lds);
+          if (tm.lane < NX) pb.S(0, is) = (M::X0K[tm.lane] >= 0) ? 0.0 : v;
+        }
+      }
+    }
+  }
   double llh = 0.0;
   int it = 0;
-  for (; it < a.nt; ++it) {
+  for (; it < a.nt && status == AMX_SUCCESS; ++it) {
     const double tout = ts[it];
     if (tout > 0.0 && bdf.tn < tout && !next_t_too_close(bdf.tn, tout)) {
       status = bdf.advance(tout);
@@ -770,8 +905,44 @@ __device__ __forceinline__ void simulate_backward(int inst, const BatchArgs& a, 
   if (status == AMX_SUCCESS && tcur > t0) {
     status = bdf.advance(t0);
     if (status == AMX_SUCCESS) {
+      xB = bdf.Z(0);
 #pragma unroll
       for (int e = 0; e < EPQ; ++e) qB[e] = bdf.ZQ(0, e);
+    }
+  }
+  // pre-equilibrated instance: sllh also gets -xB(t0) . sx0 with the
+  // implicit-function sensitivities sx0 = -J(x_ss)^{-1} df/dp of the species
+  // not reset from kfix, i.e. + v . df/dp with J(x_ss)^T v = xB(t0) on those
+  // species (SteadyStateBackwardProblem::compute_quadrature_by_lin_solve,
+  // src/backwardproblem.cpp:428-460)
+  double gss[EPQ];
+#pragma unroll
+  for (int e = 0; e < EPQ; ++e) gss[e] = 0.0;
+  if (a.kfix_pre && status == AMX_SUCCESS) {
+    const double xs = a.xss[I * NX + (tm.lane < NX ? tm.lane : 0)];
+    const double xbm = (tm.lane < NX && M::X0K[tm.lane] < 0) ? xB : 0.0;
+    LURows<NX, T> lu;
+    if (!factor_jacobian<M, true>(tm, s, pb0.ev, xs, lu, lds)) {
+      status = AMX_SINGULAR_JACOBIAN;
+    } else {
+      double v = lu.solve(xbm, tm, lds);
+      if (tm.lane >= NX) v = 0.0;
+      team_sync();
+      if (tm.lane < NX) s.v[tm.lane] = v;
+      team_sync();
+      pb0.ev.monomials();
+      pb0.ev.stoich_t();
+#pragma unroll
+      for (int e = 0; e < EPQ; ++e) {
+        const int j = tm.lane + e * T;
+        double acc = 0.0;
+        if (j < NP)
+          for (int q = M::PR_PTR[j]; q < M::PR_PTR[j + 1]; ++q) {
+            const int r = M::PR_R[q];
+            acc = fma(s.w[r], s.u[r], acc);
+          }
+        gss[e] = acc;
+      }
     }
   }
   AMX_ACC_STATS();
@@ -781,7 +952,7 @@ __device__ __forceinline__ void simulate_backward(int inst, const BatchArgs& a, 
   for (int e = 0; e < EPQ; ++e) {
     const int j = tm.lane + e * T;
     if (j < NP) {
-      const double g = -qB[e];
+      const double g = gss[e] - qB[e];
       const double pv = exp10(a.theta[I * NP + j]);
       a.sllh[I * NP + j] = (status == AMX_SUCCESS) ? (g != 0.0 ? g * pv * M_LN10 : 0.0) : NAN;
     }

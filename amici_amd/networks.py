@@ -278,6 +278,41 @@ def network_n200() -> ReactionNetwork:
                              n_observables=60, name="n200", t_end=500.0)
 
 
+def open_network(base: ReactionNetwork, seed: int, log10_kdeg_range: tuple = (-2.0, 0.0),
+                 n_synthesis: int = 5, log10_ksyn_range: tuple = (-1.0, 1.0),
+                 name: str | None = None) -> ReactionNetwork:
+    """``base`` made an open system: every species degrades (X -> 0, its own
+    rate constant, log-uniform in ``10**log10_kdeg_range``) and
+    ``n_synthesis`` species are produced at constant rates (0 -> X).  The
+    stoichiometric matrix then has full row rank (no conserved moieties), so
+    the steady state is isolated and its Jacobian is non-singular — the
+    setting of pre-equilibration with Newton's method and implicit-function
+    sensitivities (src/steadystateproblem.cpp, src/backwardproblem.cpp:394-460).
+    The input species, observables and noise model of ``base`` are kept."""
+    rng = np.random.default_rng(seed)
+    rx = list(copy.deepcopy(base.reactions))
+    names = list(base.parameter_names)
+    theta = list(base.theta_nominal)
+    for i in range(base.nx):
+        rx.append(Reaction((i,), (), len(names)))
+        names.append(f"kdeg_{base.species[i]}")
+        theta.append(rng.uniform(*log10_kdeg_range))
+    for i in np.sort(rng.choice(base.nx, size=n_synthesis, replace=False)):
+        rx.append(Reaction((), (int(i),), len(names)))
+        names.append(f"ksyn_{base.species[int(i)]}")
+        theta.append(rng.uniform(*log10_ksyn_range))
+    net = dataclasses.replace(copy.deepcopy(base), name=name or base.name + "_open", reactions=rx,
+                              parameter_names=names, theta_nominal=np.asarray(theta))
+    net.validate()
+    return net
+
+
+def network_n40_open() -> ReactionNetwork:
+    """The 40-species network with degradation of every species and synthesis
+    of five (175 reactions, 165 rate constants; seed 5)."""
+    return open_network(network_n40(), seed=5, name="n40open")
+
+
 def linear_chain(n: int = 3, seed: int = 11) -> ReactionNetwork:
     """Closed linear chain X0 <-> X1 <-> ... (first-order reactions only).
 
@@ -358,6 +393,14 @@ def condition_set(name: str) -> dict:
         return dict(network=net, theta=np.repeat(net.theta_nominal[None, :], nb, 0),
                     kfix=input_conditions(net, nb, seed=3), ts=np.linspace(0.0, 100.0, 25),
                     sigma=0.1, noise_seed=3)
+    if name == "N40P":
+        # pre-equilibrated: steady state at input level 1, then the inputs are
+        # reset to the condition's levels at t = 0
+        net = network_n40_open()
+        nb = 1024
+        return dict(network=net, theta=np.repeat(net.theta_nominal[None, :], nb, 0),
+                    kfix=input_conditions(net, nb, seed=5), kfix_preeq=np.ones((nb, net.nk)),
+                    ts=np.linspace(0.0, 100.0, 25), sigma=0.1, noise_seed=5)
     if name == "N200":
         net = network_n200()
         nb = 256

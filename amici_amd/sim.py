@@ -161,6 +161,9 @@ class Solver:
     sensi_meth: SensitivityMethod = SensitivityMethod.forward
     max_traj: int = 0
     maxtime: float = 0.0
+    newton_maxsteps: int = 40
+    ss_rtol: float = math.nan
+    ss_atol: float = math.nan
 
     def set_relative_tolerance(self, v): self.rtol = float(v)
     def get_relative_tolerance(self): return self.rtol
@@ -177,6 +180,10 @@ class Solver:
     def set_max_steps_backward_problem(self, v): self.maxstepsB = int(v)
     def set_max_time(self, v): self.maxtime = float(v)
     def get_max_time(self): return self.maxtime
+    def set_newton_max_steps(self, v): self.newton_maxsteps = int(v)
+    def get_newton_max_steps(self): return self.newton_maxsteps
+    def set_relative_tolerance_steady_state(self, v): self.ss_rtol = float(v)
+    def set_absolute_tolerance_steady_state(self, v): self.ss_atol = float(v)
     def set_sensitivity_order(self, v): self.sensi = SensitivityOrder(int(v))
     def get_sensitivity_order(self): return self.sensi
     def set_sensitivity_method(self, v): self.sensi_meth = SensitivityMethod(int(v))
@@ -197,19 +204,22 @@ class Solver:
                               atol_fsa=self.atol_fsa, rtolB=self.rtolB, atolB=self.atolB,
                               quad_rtol=self.quad_rtol, quad_atol=self.quad_atol,
                               maxstepsB=self.maxstepsB, max_traj=self.max_traj,
-                              maxtime=self.maxtime)
+                              maxtime=self.maxtime, newton_maxsteps=self.newton_maxsteps,
+                              ss_rtol=self.ss_rtol, ss_atol=self.ss_atol)
 
 
 class ExpData:
     """Per-condition inputs (include/amici/edata.h): time grid, measurements
-    [nt, ny] (NaN = missing), noise scales [nt, ny], fixed parameters and
-    optional free parameters (log10 scale)."""
+    [nt, ny] (NaN = missing), noise scales [nt, ny], fixed parameters, fixed
+    parameters of a pre-equilibration (empty = none) and optional free
+    parameters (log10 scale)."""
 
     def __init__(self, model: Model | None = None):
         self.ny = model.ny if model else 0
         self.timepoints = model.get_timepoints() if model else np.zeros(0)
         self.fixed_parameters = model.get_fixed_parameters() if model else np.zeros(0)
         self.free_parameters = np.zeros(0)
+        self.fixed_parameters_pre_equilibration = np.zeros(0)
         self._meas = np.full((len(self.timepoints), self.ny), np.nan)
         self._sigma = np.ones((len(self.timepoints), self.ny))
         self.id = ""
@@ -273,12 +283,15 @@ def _ptr(t):
 
 def simulate_batch(model: Model, solver: Solver, theta, kfix, ts, meas=None, sigma=None, *,
                    want_x: bool = True, want_y: bool = True, want_sx: bool = False,
-                   want_stats: bool = True, sequential: bool = False, stream=None) -> dict:
+                   want_stats: bool = True, sequential: bool = False, stream=None,
+                   kfix_preeq=None) -> dict:
     """Simulates ``nb`` instances in one batch; arrays may be numpy or torch.
 
     theta [nb, np] (log10), kfix [nb, nk], ts [nt] or [nb, nt], meas / sigma
     [nb, nt, ny].  Returns a dict of device tensors: x, y, sx, llh, sllh,
     status, stats.  ``sequential`` runs the plain one-instance-per-launch path.
+    ``kfix_preeq`` [nb, nk] pre-equilibrates every instance to steady state
+    with these fixed parameters before the simulation.
     """
     dev = _device()
     lib = model.lib
@@ -292,6 +305,9 @@ def simulate_batch(model: Model, solver: Solver, theta, kfix, ts, meas=None, sig
         sigma = np.ones((nb, nt, model.ny))
     sigma_t = _as_dev(sigma, dev)
     meas_t = _as_dev(meas, dev) if meas is not None else None
+    kpre_t = _as_dev(kfix_preeq, dev) if kfix_preeq is not None else None
+    if kpre_t is not None and kpre_t.shape != (nb, model.nk):
+        raise ValueError("kfix_preeq must be [nb, nk]")
     if theta_t.shape != (nb, model.np) or kfix_t.shape != (nb, model.nk):
         raise ValueError("theta / kfix shapes")
     if sigma_t.shape != (nb, nt, model.ny) or (meas_t is not None and meas_t.shape != sigma_t.shape):
@@ -313,7 +329,7 @@ def simulate_batch(model: Model, solver: Solver, theta, kfix, ts, meas=None, sig
                   if want_stats else None),
     }
     bin_ = _lib.BatchIn(nb, nt, ts_stride, _ptr(theta_t), _ptr(kfix_t) if model.nk else None,
-                        _ptr(ts_t), _ptr(meas_t), _ptr(sigma_t))
+                        _ptr(ts_t), _ptr(meas_t), _ptr(sigma_t), _ptr(kpre_t))
     bout = _lib.BatchOut(*[_ptr(out[k]) for k in ("x", "y", "sx", "llh", "sllh", "status", "stats")])
     nbytes = lib.workspace_bytes(opts, 1 if sequential else nb, nt)
     ws = model.workspace(nbytes, dev)
@@ -324,7 +340,7 @@ def simulate_batch(model: Model, solver: Solver, theta, kfix, ts, meas=None, sig
             ctypes.c_void_p(st.cuda_stream))
     if rc != 0:
         raise RuntimeError(f"amx_run_batch returned {rc}")
-    out["_keepalive"] = (theta_t, kfix_t, ts_t, meas_t, sigma_t)
+    out["_keepalive"] = (theta_t, kfix_t, ts_t, meas_t, sigma_t, kpre_t)
     return out
 
 
@@ -357,9 +373,16 @@ def run_simulations(model: Model, solver: Solver, edata_list: Sequence[ExpData],
     sigma = np.stack([e.get_noise_scales() for e in edata_list])
     if np.all(ts == ts[0:1]):
         ts = ts[0]
+    npre = {len(e.fixed_parameters_pre_equilibration) for e in edata_list}
+    if npre != {0} and npre != {model.nk}:
+        raise ValueError("pre-equilibration must be set for all conditions or for none "
+                         "(run the two groups separately)")
+    kpre = (np.stack([np.asarray(e.fixed_parameters_pre_equilibration, np.float64)
+                      for e in edata_list]).reshape(nb, model.nk) if npre == {model.nk} and model.nk
+            else None)
     meth = solver.effective_method()
     out = simulate_batch(model, solver, theta, kfix, ts, meas, sigma,
-                         want_sx=(meth == _lib.SENSI_FORWARD))
+                         want_sx=(meth == _lib.SENSI_FORWARD), kfix_preeq=kpre)
     torch.cuda.synchronize()
     host = {k: (v.cpu().numpy() if isinstance(v, torch.Tensor) else None)
             for k, v in out.items() if k != "_keepalive"}

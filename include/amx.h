@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define AMX_ABI_VERSION 1
+#define AMX_ABI_VERSION 2
 #define AMX_NSTATS 16
 
 /* sensitivity method (amici::SensitivityMethod, include/amici/defines.h) */
@@ -45,6 +45,9 @@ typedef struct {
     int max_traj;              /* adjoint: stored forward steps per instance (0 -> maxsteps + 1) */
     double maxtime;            /* wall-clock seconds per instance and problem (0 = unlimited),
                                   AMICI_MAX_TIME_EXCEEDED when exceeded (Solver::setMaxTime) */
+    int newton_maxsteps;       /* pre-equilibration: damped Newton iterations (Solver::setNewtonMaxSteps) */
+    double ss_rtol, ss_atol;   /* steady-state tolerances; NaN -> 100 x rtol / atol
+                                  (Solver ss_tol_factor, include/amici/solver.h) */
 } amx_solver_opts;
 
 typedef struct {
@@ -56,6 +59,10 @@ typedef struct {
     const double* ts;      /* [1 or nb][nt] ascending, >= t0 = 0                      */
     const double* meas;    /* [nb][nt][ny] measurements, NaN = missing; may be NULL   */
     const double* sigma;   /* [nb][nt][ny] noise standard deviations                  */
+    const double* kfix_preeq; /* [nb][nk] fixed parameters of a pre-equilibration to steady
+                                 state (ExpData::fixed_parameters_pre_equilibration), or NULL;
+                                 after it the fixed-parameter-dependent initial states are
+                                 reset from kfix                                         */
 } amx_batch_in;
 
 typedef struct {
@@ -77,7 +84,10 @@ size_t amx_workspace_bytes(const amx_solver_opts* opts, int nb, int nt);
 /* Runs the whole batch on `stream` (a hipStream_t; NULL = default stream).
  * Returns 0 when the launches were issued; per-instance results in
  * out->status after the stream completes.  Error codes:
- *   -22 invalid argument, -23 workspace too small, -30 HIP launch error. */
+ *   -22 invalid argument, -23 workspace too small, -30 HIP launch error.
+ * Pre-equilibration failures are reported per instance as
+ * AMICI_NO_STEADY_STATE (-81), AMICI_DAMPING_FACTOR_ERROR (-86) or
+ * AMICI_SINGULAR_JACOBIAN (-9987) (include/amici/defines.h). */
 int amx_run_batch(const amx_solver_opts* opts, const amx_batch_in* in, amx_batch_out* out,
                   void* workspace, size_t workspace_bytes, void* stream);
 

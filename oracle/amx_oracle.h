@@ -48,19 +48,21 @@ typedef struct {
     int ism;                 /* CV_SIMULTANEOUS=1, CV_STAGGERED=2         */
     int interp;              /* CV_HERMITE=1, CV_POLYNOMIAL=2             */
     int stldet;              /* stability limit detection (AMICI default on) */
+    double t_preeq;          /* pre-equilibration: integration end time */
 } amx_oracle_opts;
 
 /* Counters: [0]nst [1]nfe [2]nsetups [3]nje [4]nni [5]netf [6]ncfn
  *           [7]nfSe [8]nniS [9]nstB [10]nfeB [11]nsetupsB [12]nniB [13]nfQeB [14]njeB */
 #define AMX_ORACLE_NCOUNTERS 16
 
-/* One condition.  theta: [np] log10 parameters; kfix: [nk]; ts: [nt];
+/* One condition.  theta: [np] log10 parameters; kfix: [nk]; kpre: [nk]
+ * fixed parameters of a pre-equilibration (NULL = none); ts: [nt];
  * meas/sigma: [nt*ny] (NaN measurement = missing).
  * Outputs (may be NULL): x [nt*nx], y [nt*ny], sx [nt*np*nx] (FSA only),
  * llh [1], sllh [np] (log10 scale), counters [AMX_ORACLE_NCOUNTERS].
  * Returns AMICI status (0 success, <0 failure as in include/amici/defines.h). */
 int amx_oracle_simulate(const amx_oracle_model* m, const amx_oracle_opts* o,
-                        const double* theta, const double* kfix, int nt,
+                        const double* theta, const double* kfix, const double* kpre, int nt,
                         const double* ts, const double* meas, const double* sigma,
                         double* x, double* y, double* sx, double* llh,
                         double* sllh, long* counters);
@@ -69,7 +71,7 @@ int amx_oracle_simulate(const amx_oracle_model* m, const amx_oracle_opts* o,
  * Per-condition arrays are strided by their single-condition size. */
 int amx_oracle_simulate_batch(const amx_oracle_model* m, const amx_oracle_opts* o,
                               int nb, const double* theta, const double* kfix,
-                              int nt, const double* ts, const double* meas,
+                              const double* kpre, int nt, const double* ts, const double* meas,
                               const double* sigma, double* x, double* y,
                               double* sx, double* llh, double* sllh,
                               long* counters, int* status, int nthreads);

@@ -49,6 +49,7 @@
 #define AMICI_SUCCESS 0
 #define AMICI_ERROR (-99)
 #define AMICI_TOO_MUCH_WORK (-1)
+#define AMICI_NO_STEADY_STATE (-81) /* include/amici/defines.h */
 
 typedef struct {
     const amx_oracle_model* m;
@@ -295,8 +296,65 @@ static int reinit_post_process_b(void* cvode_mem, N_Vector xB, double tnext) {
         if (_s < 0) { status = _s; goto cleanup; }   \
     } while (0)
 
+/* Pre-equilibration (SteadyStateProblem, src/forwardproblem.cpp:755-1223,
+ * restated by integration instead of Newton's method): CVODES from
+ * x0(kpre) to t = o->t_preeq, with forward sensitivities from sx0 = 0 when
+ * sx != NULL; the state there must satisfy AMICI's steady-state criterion
+ * WRMS(xdot) < 1 with tolerances 100 x (rtol, atol) (solver.h ss_tol_factor).
+ * On return x holds x_ss and sx (if given) dx_ss/dp (linear parameters). */
+static int preequilibrate(const amx_oracle_model* m, const amx_oracle_opts* o, ud_t* ud,
+                          const double* kpre, N_Vector x, N_Vector* sx, SUNContext ctx) {
+    const int nx = m->nx, np = m->np;
+    int status = AMICI_SUCCESS;
+    void* cv = NULL;
+    SUNMatrix A = NULL;
+    SUNLinearSolver LS = NULL;
+    int* plist = (int*)malloc(sizeof(int) * (size_t)(np > 0 ? np : 1));
+    double* atols = (double*)malloc(sizeof(double) * (size_t)(np > 0 ? np : 1));
+    for (int j = 0; j < np; ++j) { plist[j] = j; atols[j] = o->atol; }
+    double* xd = N_VGetArrayPointer(x);
+    for (int i = 0; i < nx; ++i) xd[i] = (m->x0k[i] >= 0) ? kpre[m->x0k[i]] : m->x0c[i];
+    cv = CVodeCreate(CV_BDF, ctx);
+    CHECK(CVodeInit(cv, cb_f, 0.0, x));
+    CHECK(CVodeSStolerances(cv, o->rtol, o->atol));
+    CHECK(CVodeSetUserData(cv, ud));
+    A = SUNDenseMatrix(nx, nx, ctx);
+    LS = SUNLinSol_Dense(x, A, ctx);
+    CHECK(CVodeSetLinearSolver(cv, LS, A));
+    CHECK(CVodeSetJacFn(cv, cb_jac));
+    CHECK(CVodeSetMaxNumSteps(cv, o->maxsteps));
+    if (sx) {
+        for (int j = 0; j < np; ++j) N_VConst(0.0, sx[j]);
+        CHECK(CVodeSensInit1(cv, np, o->ism, cb_fS1, sx));
+        CHECK(CVodeSetSensParams(cv, (double*)ud->p, NULL, plist));
+        CHECK(CVodeSensSStolerances(cv, o->rtol, atols));
+        CHECK(CVodeSetSensErrCon(cv, SUNTRUE));
+    }
+    {
+        double t = 0.0;
+        int rc = CVode(cv, o->t_preeq, x, &t, CV_NORMAL);
+        if (rc < 0) { status = rc; goto cleanup; }
+        if (sx) CHECK(CVodeGetSens(cv, &t, sx));
+        double* f = (double*)malloc(sizeof(double) * (size_t)nx);
+        amx_oracle_xdot(m, ud->p, xd, f);
+        double acc = 0.0;
+        for (int i = 0; i < nx; ++i) {
+            const double e = f[i] / (100.0 * o->rtol * fabs(xd[i]) + 100.0 * o->atol);
+            acc += e * e;
+        }
+        free(f);
+        if (!(sqrt(acc / nx) < 1.0)) status = AMICI_NO_STEADY_STATE;
+    }
+cleanup:
+    if (cv) CVodeFree(&cv);
+    if (LS) SUNLinSolFree(LS);
+    if (A) SUNMatDestroy(A);
+    free(plist); free(atols);
+    return status;
+}
+
 int amx_oracle_simulate(const amx_oracle_model* m, const amx_oracle_opts* o,
-                        const double* theta, const double* kfix, int nt,
+                        const double* theta, const double* kfix, const double* kpre, int nt,
                         const double* ts, const double* meas, const double* sigma,
                         double* x_out, double* y_out, double* sx_out, double* llh_out,
                         double* sllh_out, long* counters) {
@@ -317,6 +375,7 @@ int amx_oracle_simulate(const amx_oracle_model* m, const amx_oracle_opts* o,
     double* dJydx = (double*)calloc((size_t)nt * nx, sizeof(double));
     double* sllh = (double*)calloc((size_t)np, sizeof(double));
     double* ybuf = (double*)malloc(sizeof(double) * (size_t)(ny > 0 ? ny : 1));
+    double* sx0 = (double*)calloc((size_t)nx * (np > 0 ? np : 1), sizeof(double)); /* [np][nx] */
     int* plist = (int*)malloc(sizeof(int) * (size_t)(np > 0 ? np : 1));
     double* atols = (double*)malloc(sizeof(double) * (size_t)(np > 0 ? np : 1));
     ud_t ud;
@@ -334,7 +393,23 @@ int amx_oracle_simulate(const amx_oracle_model* m, const amx_oracle_opts* o,
 
     if (SUNContext_Create(SUN_COMM_NULL, &ctx) != 0) { status = AMICI_ERROR; goto cleanup; }
     x = N_VNew_Serial(nx, ctx);
-    {
+    if (kpre) {
+        /* pre-equilibration, then the fixed-parameter-dependent initial states
+         * are re-initialised from kfix (reinitialize_fixed_parameter_initial_states) */
+        N_Vector* sxp = (o->sensi >= 1 && np > 0) ? N_VCloneVectorArray(np, x) : NULL;
+        status = preequilibrate(m, o, &ud, kpre, x, sxp, ctx);
+        if (sxp) {
+            for (int j = 0; j < np; ++j) {
+                const double* s = N_VGetArrayPointer(sxp[j]);
+                for (int i = 0; i < nx; ++i) sx0[(size_t)j * nx + i] = (m->x0k[i] >= 0) ? 0.0 : s[i];
+            }
+            N_VDestroyVectorArray(sxp, np);
+        }
+        if (status != AMICI_SUCCESS) goto cleanup;
+        double* xd = N_VGetArrayPointer(x);
+        for (int i = 0; i < nx; ++i)
+            if (m->x0k[i] >= 0) xd[i] = kfix[m->x0k[i]];
+    } else {
         double* xd = N_VGetArrayPointer(x);
         for (int i = 0; i < nx; ++i) xd[i] = (m->x0k[i] >= 0) ? kfix[m->x0k[i]] : m->x0c[i];
     }
@@ -351,7 +426,8 @@ int amx_oracle_simulate(const amx_oracle_model* m, const amx_oracle_opts* o,
     CHECK(CVodeSetMaxConvFails(cv, 10));
     if (o->sensi == 1 && np > 0) {
         sx = N_VCloneVectorArray(np, x);
-        for (int j = 0; j < np; ++j) N_VConst(0.0, sx[j]); /* sx0 = dx0/dp = 0 */
+        for (int j = 0; j < np; ++j) /* sx0 = dx0/dp: 0, or from pre-equilibration */
+            memcpy(N_VGetArrayPointer(sx[j]), sx0 + (size_t)j * nx, sizeof(double) * (size_t)nx);
         CHECK(CVodeSensInit1(cv, np, o->ism, cb_fS1, sx));
         CHECK(CVodeSetSensParams(cv, p, NULL, plist));
         CHECK(CVodeSensSStolerances(cv, o->rtol, atols));
@@ -500,9 +576,15 @@ int amx_oracle_simulate(const amx_oracle_model* m, const amx_oracle_opts* o,
                 CVodeGetNumJacEvals(cvB, &v); counters[14] = v;
             }
         }
-        /* rdata.cpp:514-629 with sx0 = dx0/dp = 0 (x0 depends on fixed parameters only) */
+        /* rdata.cpp:514-629: sllh = -(xB(t0) . sx0 + xQB); sx0 = 0 unless the
+         * instance was pre-equilibrated */
         const double* qd = N_VGetArrayPointer(qB);
-        for (int j = 0; j < np; ++j) sllh[j] -= 0.0 + qd[j];
+        const double* bd = N_VGetArrayPointer(xB);
+        for (int j = 0; j < np; ++j) {
+            double xs = 0.0;
+            for (int i = 0; i < nx; ++i) xs += bd[i] * sx0[(size_t)j * nx + i];
+            sllh[j] -= xs + qd[j];
+        }
     }
 
     /* rdata.cpp:782-830: log10 chain rule on non-zero entries */
@@ -532,13 +614,13 @@ cleanup:
     if (xB) N_VDestroy(xB);
     if (qB) N_VDestroy(qB);
     if (ctx) SUNContext_Free(&ctx);
-    free(p); free(work); free(dJydx); free(sllh); free(ybuf); free(plist); free(atols);
+    free(p); free(work); free(dJydx); free(sllh); free(ybuf); free(plist); free(atols); free(sx0);
     return status;
 }
 
 int amx_oracle_simulate_batch(const amx_oracle_model* m, const amx_oracle_opts* o,
                               int nb, const double* theta, const double* kfix,
-                              int nt, const double* ts, const double* meas,
+                              const double* kpre, int nt, const double* ts, const double* meas,
                               const double* sigma, double* x, double* y,
                               double* sx, double* llh, double* sllh,
                               long* counters, int* status, int nthreads) {
@@ -548,7 +630,8 @@ int amx_oracle_simulate_batch(const amx_oracle_model* m, const amx_oracle_opts* 
     for (int b = 0; b < nb; ++b) {
         const size_t B = (size_t)b;
         int s = amx_oracle_simulate(
-            m, o, theta + B * np, kfix + B * nk, nt, ts + B * (size_t)nt,
+            m, o, theta + B * np, kfix + B * nk, kpre ? kpre + B * nk : NULL, nt,
+            ts + B * (size_t)nt,
             meas ? meas + B * (size_t)nt * ny : NULL, sigma + B * (size_t)nt * ny,
             x ? x + B * (size_t)nt * nx : NULL, y ? y + B * (size_t)nt * ny : NULL,
             sx ? sx + B * (size_t)nt * np * nx : NULL, llh ? llh + B : NULL,

@@ -37,7 +37,7 @@ class _Opts(ctypes.Structure):
     _fields_ = [("rtol", ctypes.c_double), ("atol", ctypes.c_double),
                 ("quad_rtol", ctypes.c_double), ("quad_atol", ctypes.c_double),
                 ("maxsteps", ctypes.c_long), ("sensi", ctypes.c_int), ("ism", ctypes.c_int),
-                ("interp", ctypes.c_int), ("stldet", ctypes.c_int)]
+                ("interp", ctypes.c_int), ("stldet", ctypes.c_int), ("t_preeq", ctypes.c_double)]
 
 
 def build(reference: str = "/root/reference/ThirdParty/sundials") -> Path:
@@ -105,9 +105,13 @@ class OracleModel:
 
 def simulate(network, theta, kfix, ts, meas=None, sigma=None, *, rtol=1e-10, atol=1e-14,
              sensi=0, ism=CV_SIMULTANEOUS, interp=CV_POLYNOMIAL, stldet=1, maxsteps=100000,
-             quad_rtol=1e-8, quad_atol=1e-12, nthreads=1, want_sx=False):
+             quad_rtol=1e-8, quad_atol=1e-12, nthreads=1, want_sx=False, kpre=None,
+             t_preeq=1e6):
     """CVODES-oracle batch: theta [nb, np], kfix [nb, nk], ts [nt] or [nb, nt],
     meas / sigma [nb, nt, ny].  sensi: 0 none, 1 forward, 2 adjoint.
+    kpre [nb, nk]: fixed parameters of a pre-equilibration to steady state
+    (integration to t_preeq), after which the fixed-parameter-dependent
+    initial states are reset from kfix.
     Returns dict of numpy arrays x, y, sx, llh, sllh, status, counters."""
     om = OracleModel(network)
     theta = np.ascontiguousarray(np.atleast_2d(theta), np.float64)
@@ -128,9 +132,11 @@ def simulate(network, theta, kfix, ts, meas=None, sigma=None, *, rtol=1e-10, ato
     sllh = np.zeros((nb, network.np))
     counters = np.zeros((nb, NCOUNTERS), np.int64)
     status = np.zeros(nb, np.int32)
-    opts = _Opts(rtol, atol, quad_rtol, quad_atol, maxsteps, sensi, ism, interp, stldet)
+    opts = _Opts(rtol, atol, quad_rtol, quad_atol, maxsteps, sensi, ism, interp, stldet, t_preeq)
+    if kpre is not None:
+        kpre = np.ascontiguousarray(np.asarray(kpre, np.float64).reshape(nb, network.nk))
     lib().amx_oracle_simulate_batch(
-        ctypes.byref(om.c), ctypes.byref(opts), ctypes.c_int(nb), _p(theta), _p(kfix),
+        ctypes.byref(om.c), ctypes.byref(opts), ctypes.c_int(nb), _p(theta), _p(kfix), _p(kpre),
         ctypes.c_int(nt), _p(ts), _p(meas), _p(sigma), _p(x), _p(y), _p(sx), _p(llh), _p(sllh),
         _p(counters), _p(status), ctypes.c_int(nthreads))
     return dict(x=x, y=y, sx=sx, llh=llh, sllh=sllh if sensi else None, status=status,

@@ -115,8 +115,28 @@ def main() -> None:
                         meas=meas_l, sigma=sig_l, x=f["x"], y=f["y"], sx=f["sx"], llh=f["llh"],
                         sllh_fsa=f["sllh"], sllh_asa=a["sllh"])
     n40_counters()
+    n40_preeq()
     for p in sorted(OUT.glob("*.npz")):
         print(p.name, p.stat().st_size)
+
+
+def n40_preeq() -> None:
+    """16 conditions of the pre-equilibrated open 40-species network: steady
+    state at input level 1 (CVODES integrated to t = 1e6 with forward
+    sensitivities), inputs reset to the condition's levels, then 25 outputs;
+    forward and adjoint gradients through the pre-equilibration."""
+    cs = networks.condition_set("N40P")
+    net, ts = cs["network"], cs["ts"]
+    nb = 16
+    theta, kfix, kpre = cs["theta"][:nb], cs["kfix"][:nb], cs["kfix_preeq"][:nb]
+    y = O.simulate(net, theta, kfix, ts, nthreads=8, kpre=kpre, **TIGHT)["y"]
+    meas = networks.add_noise(y, net.noise, cs["sigma"], seed=cs["noise_seed"])
+    sigma = np.full_like(meas, cs["sigma"])
+    f = run(net, theta, kfix, ts, meas, sigma, 1, TIGHT, kpre=kpre, want_sx=True)
+    a = run(net, theta, kfix, ts, meas, sigma, 2, TIGHT, kpre=kpre)
+    np.savez_compressed(OUT / "n40open_preeq16.npz", theta=theta, kfix=kfix, kfix_preeq=kpre,
+                        ts=ts, meas=meas, sigma=sigma, x=f["x"], y=f["y"], llh=f["llh"],
+                        sllh_fsa=f["sllh"], sllh_asa=a["sllh"], sx0=f["sx"][:2, 0])
 
 
 def n40_counters() -> None:
@@ -131,5 +151,7 @@ def n40_counters() -> None:
 if __name__ == "__main__":
     if sys.argv[1:] == ["n40_counters"]:
         n40_counters()
+    elif sys.argv[1:] == ["n40_preeq"]:
+        n40_preeq()
     else:
         main()

@@ -21,10 +21,11 @@ def _declared():
     return sorted(set(re.findall(r"^\s*(?:int|size_t|const char\*)\s+(amx_\w+)\s*\(", text, re.M)))
 
 
-@pytest.fixture(scope="module", params=["robertson", "n40", "linear3"])
+@pytest.fixture(scope="module", params=["robertson", "n40", "linear3", "n40open"])
 def lib(request):
     net = {"robertson": networks.robertson, "n40": networks.network_n40,
-           "linear3": lambda: networks.linear_chain(3)}[request.param]()
+           "linear3": lambda: networks.linear_chain(3),
+           "n40open": networks.network_n40_open}[request.param]()
     so = codegen.compile_model(net)
     return net, _lib.ModelLib(so)
 
@@ -61,22 +62,29 @@ def test_invalid_arguments_rejected_on_host(lib):
     net, L = lib
     fn = L.lib.amx_run_batch
     o = _lib.make_opts(sensi_meth=_lib.SENSI_FORWARD)
-    ins = _lib.BatchIn(1, 3, 0, None, None, None, None, None)
+    ins = _lib.BatchIn(1, 3, 0, None, None, None, None, None, None)
     outs = _lib.BatchOut()
     assert fn(ctypes.byref(o), ctypes.byref(ins), ctypes.byref(outs), None, 0, None) == -22
     assert fn(None, ctypes.byref(ins), ctypes.byref(outs), None, 0, None) == -22
-    empty = _lib.BatchIn(0, 3, 0, None, None, None, None, None)
+    empty = _lib.BatchIn(0, 3, 0, None, None, None, None, None, None)
     assert fn(ctypes.byref(o), ctypes.byref(empty), ctypes.byref(outs), None, 0, None) == 0
     # a non-null but too-small workspace is refused before any launch
     buf = (ctypes.c_double * 4)()
     p = ctypes.cast(buf, ctypes.c_void_p).value
-    ins = _lib.BatchIn(1, 3, 0, p, p if net.nk else None, p, None, p)
+    ins = _lib.BatchIn(1, 3, 0, p, p if net.nk else None, p, None, p, None)
     outs = _lib.BatchOut(None, None, None, p, p, p, None)
     assert fn(ctypes.byref(o), ctypes.byref(ins), ctypes.byref(outs), p, 8, None) == -23
     bad = _lib.make_opts(rtol=-1.0)
     assert fn(ctypes.byref(bad), ctypes.byref(ins), ctypes.byref(outs), p, 8, None) == -22
     bad = _lib.make_opts(rtol=math.nan)
     assert fn(ctypes.byref(bad), ctypes.byref(ins), ctypes.byref(outs), p, 8, None) == -22
+    # pre-equilibration needs fixed parameters and a non-negative Newton budget
+    pre = _lib.BatchIn(1, 3, 0, p, p if net.nk else None, p, None, p, p)
+    if net.nk == 0:
+        assert fn(ctypes.byref(o), ctypes.byref(pre), ctypes.byref(outs), p, 1 << 30, None) == -22
+    neg = _lib.make_opts(sensi_meth=_lib.SENSI_FORWARD, newton_maxsteps=-1)
+    if net.nk:
+        assert fn(ctypes.byref(neg), ctypes.byref(pre), ctypes.byref(outs), p, 1 << 30, None) == -22
     # the sequential path has no adjoint
     adj = _lib.make_opts(sensi_meth=_lib.SENSI_ADJOINT)
     assert L.lib.amx_run_sequential(ctypes.byref(adj), ctypes.byref(ins), ctypes.byref(outs),

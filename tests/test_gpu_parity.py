@@ -149,3 +149,37 @@ def test_robertson_from_sbml(gpu):
     assert (out["status"] == 0).all(), out["status"]
     _close(out["x"], g["x"], Y_RTOL, Y_FLOOR, "x")
     _close(out["sllh"], g["sllh_fsa"], 1e-5, 1e-7, "sllh (forward)")
+
+
+@pytest.fixture(scope="module")
+def open_model(gpu):
+    from amici_amd import sim
+    return sim.Model(networks.network_n40_open())
+
+
+@pytest.mark.parametrize("meth", [0, 1, 2])
+def test_preequilibration(open_model, meth):
+    """Pre-equilibration to steady state (damped Newton; implicit-function
+    sensitivities) against the oracle's CVODES integration to t = 1e6 with
+    forward sensitivities, for states, forward and adjoint gradients."""
+    import torch
+    from amici_amd import sim
+    g = _load("n40open_preeq16")
+    sol = sim.Solver(**TIGHT)
+    if meth:
+        sol.set_sensitivity_order(sim.SensitivityOrder.first)
+        sol.set_sensitivity_method(meth)
+    out = sim.simulate_batch(open_model, sol, g["theta"], g["kfix"], g["ts"], g["meas"], g["sigma"],
+                             kfix_preeq=g["kfix_preeq"], want_sx=(meth == 1))
+    torch.cuda.synchronize()
+    out = {k: (v.cpu().numpy() if hasattr(v, "cpu") else v) for k, v in out.items()
+           if k != "_keepalive"}
+    _dump(f"preeq_{meth}", out)
+    assert (out["status"] == 0).all(), out["status"]
+    _close(out["x"], g["x"], Y_RTOL, Y_FLOOR, "x")
+    _check_llh(open_model.network, out, g)
+    if meth == 1:
+        _close(out["sx"][:2, 0], g["sx0"], 1e-5, 1e-7, "sx0 (implicit function)")
+    if meth:
+        key = "sllh_fsa" if meth == 1 else "sllh_asa"
+        _close(out["sllh"], g[key], 1e-5, 1e-7, f"sllh ({meth})")

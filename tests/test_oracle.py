@@ -215,3 +215,55 @@ def test_batch_threads_deterministic():
     a = O.simulate(net, th, np.zeros((16, 0)), networks.robertson_times(), nthreads=1, rtol=1e-8, atol=1e-12)
     b = O.simulate(net, th, np.zeros((16, 0)), networks.robertson_times(), nthreads=4, rtol=1e-8, atol=1e-12)
     np.testing.assert_array_equal(a["x"], b["x"])
+
+
+def test_preequilibration_steady_state_and_sensitivities():
+    """Pre-equilibration of the open network: the oracle's steady state (CVODES
+    to t = 1e6) against scipy Radau, its integrated sensitivities against the
+    implicit-function solution -J^{-1} df/dp (steadystateproblem.cpp), and the
+    inputs reset from kfix afterwards."""
+    net = networks.network_n40_open()
+    cs = networks.condition_set("N40P")
+    om = O.OracleModel(net)
+    th = cs["theta"][0]
+    inp = net.x0_fixed >= 0
+    x0 = np.where(inp, cs["kfix_preeq"][0][np.maximum(net.x0_fixed, 0)], net.x0_const)
+    sol = solve_ivp(lambda t, y: om.xdot(th, y), (0, 1e6), x0, method="Radau", rtol=1e-12,
+                    atol=1e-16, jac=lambda t, y: om.jac(th, y))
+    xs = sol.y[:, -1]
+    r = O.simulate(net, th[None], cs["kfix"][:1], np.array([0.0]), rtol=1e-10, atol=1e-14,
+                   kpre=cs["kfix_preeq"][:1], sensi=1, want_sx=True)
+    assert r["status"][0] == 0
+    xo = r["x"][0, 0]
+    np.testing.assert_allclose(xo[~inp], xs[~inp], rtol=1e-8, atol=1e-14)
+    np.testing.assert_array_equal(xo[inp], cs["kfix"][0][net.x0_fixed[inp]])
+    sx_ift = -np.linalg.solve(om.jac(th, xs), om.dxdotdp(th, xs)) * (10.0 ** th) * math.log(10.0)
+    sx_ift[inp] = 0.0
+    np.testing.assert_allclose(r["sx"][0, 0].T, sx_ift, rtol=1e-6, atol=1e-9 * np.abs(sx_ift).max())
+
+
+def test_preequilibration_gradients():
+    """Forward and adjoint gradients through pre-equilibration agree with each
+    other and with central finite differences of llh (log10 scale)."""
+    net = networks.network_n40_open()
+    cs = networks.condition_set("N40P")
+    b = slice(3, 4)
+    kw = dict(rtol=1e-11, atol=1e-15, kpre=cs["kfix_preeq"][b])
+    th = cs["theta"][b].copy()
+    y = O.simulate(net, th, cs["kfix"][b], cs["ts"], **kw)["y"]
+    meas = networks.add_noise(y, net.noise, 0.1, seed=8)
+    sig = np.full_like(meas, 0.1)
+    f = O.simulate(net, th, cs["kfix"][b], cs["ts"], meas, sig, sensi=1, **kw)
+    a = O.simulate(net, th, cs["kfix"][b], cs["ts"], meas, sig, sensi=2, quad_rtol=1e-11,
+                   quad_atol=1e-15, **kw)
+    assert f["status"][0] == 0 and a["status"][0] == 0
+    scale = np.abs(f["sllh"][0]).max()
+    np.testing.assert_allclose(a["sllh"][0], f["sllh"][0], rtol=1e-6, atol=1e-7 * scale)
+    eps = 1e-4
+    for j in np.argsort(-np.abs(f["sllh"][0]))[:3]:
+        tp, tm = th.copy(), th.copy()
+        tp[0, j] += eps / 2
+        tm[0, j] -= eps / 2
+        lp = O.simulate(net, tp, cs["kfix"][b], cs["ts"], meas, sig, **kw)["llh"][0]
+        lm = O.simulate(net, tm, cs["kfix"][b], cs["ts"], meas, sig, **kw)["llh"][0]
+        assert f["sllh"][0][j] == pytest.approx((lp - lm) / eps, rel=1e-4, abs=1e-6 * scale)
