@@ -60,6 +60,7 @@ enum : int {
   AMX_RHSFUNC_FAIL = -8,
   AMX_ILL_INPUT = -22,
   AMX_TOO_CLOSE = -27,
+  AMX_ERROR = -99,
   AMX_MAX_TIME_EXCEEDED = -1000,
   AMX_NO_STEADY_STATE = -81,
   AMX_DAMPING_FACTOR_ERROR = -86,

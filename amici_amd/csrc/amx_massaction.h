@@ -791,6 +791,13 @@ __device__ __forceinline__ void simulate_forward(int inst, const BatchArgs& a, c
   for (; it < a.nt && status == AMX_SUCCESS; ++it) {
     const double tout = ts[it];
     if (tout > 0.0 && bdf.tn < tout && !next_t_too_close(bdf.tn, tout)) {
+      // the step budget counts over the whole forward run; one used up
+      // exactly at an output time is an exception in Solver::apply_max_num_steps
+      // (src/solver.cpp:107-119), reported as AMICI_ERROR (src/amici.cpp:156-157)
+      if (bdf.nst >= bdf.sp.maxsteps) {
+        status = AMX_ERROR;
+        break;
+      }
       status = bdf.advance(tout);
       if (status != AMX_SUCCESS) break;
     }

@@ -20,8 +20,9 @@ from typing import Sequence
 import numpy as np
 import torch
 
-from . import _lib
+from . import _lib, rdata as _rdata
 from .networks import ReactionNetwork
+from .rdata import RDataReporting
 
 AMICI_SUCCESS = 0
 AMICI_NOT_RUN = -1001
@@ -61,6 +62,8 @@ class Model:
         self._k = np.asarray(network.k_nominal, np.float64).copy()
         self._ts = np.zeros(0)
         self._ws: dict = {}
+        self._sigma_res = False
+        self._min_sigma = 50.0
 
     # dimensions
     @property
@@ -131,6 +134,20 @@ class Model:
     def nt(self) -> int:
         return len(self._ts)
 
+    # error residuals sqrt(2 log sigma + min_sigma) appended to res
+    # (Model::set_add_sigma_residuals, include/amici/model.h:885-929)
+    def set_add_sigma_residuals(self, v: bool) -> None:
+        self._sigma_res = bool(v)
+
+    def get_add_sigma_residuals(self) -> bool:
+        return self._sigma_res
+
+    def set_minimum_sigma_residuals(self, v: float) -> None:
+        self._min_sigma = float(v)
+
+    def get_minimum_sigma_residuals(self) -> float:
+        return self._min_sigma
+
     def create_solver(self) -> "Solver":
         return Solver()
 
@@ -164,6 +181,7 @@ class Solver:
     newton_maxsteps: int = 40
     ss_rtol: float = math.nan
     ss_atol: float = math.nan
+    rdata_reporting: RDataReporting = RDataReporting.full
 
     def set_relative_tolerance(self, v): self.rtol = float(v)
     def get_relative_tolerance(self): return self.rtol
@@ -184,6 +202,8 @@ class Solver:
     def get_newton_max_steps(self): return self.newton_maxsteps
     def set_relative_tolerance_steady_state(self, v): self.ss_rtol = float(v)
     def set_absolute_tolerance_steady_state(self, v): self.ss_atol = float(v)
+    def set_return_data_reporting_mode(self, v): self.rdata_reporting = RDataReporting(int(v))
+    def get_return_data_reporting_mode(self): return self.rdata_reporting
     def set_sensitivity_order(self, v): self.sensi = SensitivityOrder(int(v))
     def get_sensitivity_order(self): return self.sensi
     def set_sensitivity_method(self, v): self.sensi_meth = SensitivityMethod(int(v))
@@ -214,7 +234,33 @@ class ExpData:
     parameters of a pre-equilibration (empty = none) and optional free
     parameters (log10 scale)."""
 
-    def __init__(self, model: Model | None = None):
+    def __init__(self, source: "Model | ReturnData | None" = None, sigma_y=None,
+                 sigma_z: float = 0.0, seed: int = -1):
+        """``ExpData(model)``: empty data on the model's time grid.
+        ``ExpData(rdata, sigma_y, sigma_z, seed)``: synthetic data from a
+        simulation, y + N(0, sigma_y) with noise scales sigma_y (scalar,
+        [ny] or [nt, ny]; src/edata.cpp:74-134; seed < 0 draws a random
+        seed).  There are no event observables, so sigma_z is unused."""
+        del sigma_z
+        if isinstance(source, ReturnData):
+            if source.y is None:
+                raise ValueError("the ReturnData holds no observables")
+            self.ny = source.y.shape[1]
+            self.timepoints = np.asarray(source.ts, np.float64).copy()
+            self.fixed_parameters = np.zeros(0)
+            self.free_parameters = np.zeros(0)
+            self.fixed_parameters_pre_equilibration = np.zeros(0)
+            nt = len(self.timepoints)
+            sig = np.broadcast_to(np.asarray(1.0 if sigma_y is None else sigma_y, np.float64),
+                                  (nt, self.ny)).copy()
+            if not np.all(sig > 0):
+                raise ValueError("sigma_y must be positive")
+            rng = np.random.default_rng(None if seed < 0 else seed)
+            self._meas = source.y + sig * rng.standard_normal((nt, self.ny))
+            self._sigma = sig
+            self.id = ""
+            return
+        model = source
         self.ny = model.ny if model else 0
         self.timepoints = model.get_timepoints() if model else np.zeros(0)
         self.fixed_parameters = model.get_fixed_parameters() if model else np.zeros(0)
@@ -250,7 +296,9 @@ class ExpData:
 
 @dataclass
 class ReturnData:
-    """Results of one condition (fields and shapes of ReturnDataView)."""
+    """Results of one condition (fields and shapes of ReturnDataView,
+    python/sdist/amici/sim/sundials/_numpy.py:412-450); fields the reporting
+    mode leaves out are None."""
 
     ts: np.ndarray
     x: np.ndarray | None = None
@@ -258,8 +306,14 @@ class ReturnData:
     sigmay: np.ndarray | None = None
     sx: np.ndarray | None = None
     sy: np.ndarray | None = None
+    ssigmay: np.ndarray | None = None
     llh: float = math.nan
     sllh: np.ndarray | None = None
+    chi2: float = math.nan
+    res: np.ndarray | None = None
+    sres: np.ndarray | None = None
+    FIM: np.ndarray | None = None
+    sigma_res: bool = False
     status: int = AMICI_NOT_RUN
     numsteps: int = 0
     numrhsevals: int = 0
@@ -381,24 +435,53 @@ def run_simulations(model: Model, solver: Solver, edata_list: Sequence[ExpData],
                       for e in edata_list]).reshape(nb, model.nk) if npre == {model.nk} and model.nk
             else None)
     meth = solver.effective_method()
-    out = simulate_batch(model, solver, theta, kfix, ts, meas, sigma,
-                         want_sx=(meth == _lib.SENSI_FORWARD), kfix_preeq=kpre)
+    fsa = meth == _lib.SENSI_FORWARD
+    fields = _rdata.reported_fields(solver.rdata_reporting, forward=fsa,
+                                    sensi=meth != _lib.SENSI_NONE)
+    out = simulate_batch(model, solver, theta, kfix, ts, meas, sigma, want_x="x" in fields,
+                         want_sx=fsa, kfix_preeq=kpre)
+    dev = out["llh"].device
+    meas_t, sigma_t = out["_keepalive"][3], out["_keepalive"][4]
+    sy_t = None
+    if out["sx"] is not None:
+        C = torch.as_tensor(model.network.observable_matrix, dtype=torch.float64, device=dev)
+        sy_t = out["sx"] @ C.T                                     # [nb, nt, np, ny]
+    derived = _rdata.derived_outputs(out["y"], sy_t, meas_t, sigma_t, out["status"],
+                                     model.network.noise, sigma_res=model.get_add_sigma_residuals(),
+                                     sigma_offset=model.get_minimum_sigma_residuals(), fields=fields)
     torch.cuda.synchronize()
     host = {k: (v.cpu().numpy() if isinstance(v, torch.Tensor) else None)
             for k, v in out.items() if k != "_keepalive"}
-    C = model.network.observable_matrix
+    host.update({k: v.cpu().numpy() for k, v in derived.items()})
+    if sy_t is not None:
+        host["sy"] = sy_t.cpu().numpy()
     res = []
     for b, e in enumerate(edata_list):
         st = host["stats"][b]
-        rd = ReturnData(ts=np.asarray(e.timepoints), x=host["x"][b], y=host["y"][b],
-                        sigmay=sigma[b], llh=float(host["llh"][b]), status=int(host["status"][b]),
+        rd = ReturnData(ts=np.asarray(e.timepoints), status=int(host["status"][b]),
                         numsteps=int(st[0]), numrhsevals=int(st[1]), numstepsB=int(st[8]),
-                        stats=st.astype(np.int64))
-        if host["sllh"] is not None:
+                        stats=st.astype(np.int64), sigma_res=model.get_add_sigma_residuals())
+        if "x" in fields:
+            rd.x = host["x"][b]
+        if "y" in fields:
+            rd.y = host["y"][b]
+            rd.sigmay = sigma[b]
+        if "llh" in fields:
+            rd.llh = float(host["llh"][b])
+        if "sllh" in fields and host["sllh"] is not None:
             rd.sllh = host["sllh"][b]
-        if host["sx"] is not None:
+        if "sx" in fields:
             rd.sx = host["sx"][b]
-            rd.sy = np.einsum("tpx,yx->tpy", rd.sx, C)
+        if "sy" in fields:
+            rd.sy = host["sy"][b]
+            rd.ssigmay = host["ssigmay"][b]
+        if "res" in fields:
+            rd.res = host["res"][b]
+            rd.chi2 = float(host["chi2"][b])
+        if "sres" in fields:
+            rd.sres = host["sres"][b]
+        if "FIM" in fields:
+            rd.FIM = host["FIM"][b]
         if rd.status != AMICI_SUCCESS:
             rd.messages.append(model.lib.status_string(rd.status))
         res.append(rd)

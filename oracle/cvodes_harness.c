@@ -447,7 +447,9 @@ int amx_oracle_simulate(const amx_oracle_model* m, const amx_oracle_opts* o,
                 CHECK(CVodeSetStopTime(cv, tout));
                 long cur = 0;
                 CVodeGetNumSteps(cv, &cur);
-                if (o->maxsteps <= cur) { status = AMICI_TOO_MUCH_WORK; goto cleanup; }
+                /* budget used up at an output time: AmiException in
+                   Solver::apply_max_num_steps (src/solver.cpp:107-119) -> AMICI_ERROR */
+                if (o->maxsteps <= cur) { status = AMICI_ERROR; goto cleanup; }
                 CHECK(CVodeSetMaxNumSteps(cv, o->maxsteps - cur));
                 int s = (o->sensi == 2 && np > 0)
                             ? CVodeF(cv, tout, x, &t, CV_NORMAL, &ncheck)

@@ -148,9 +148,35 @@ def n40_counters() -> None:
                         llh=r["llh"])
 
 
+def step_limit() -> None:
+    """Robertson at the nominal parameters with a forward step budget
+    (Solver::set_max_steps): the budget counts over the whole run
+    (Solver::apply_max_num_steps, src/solver.cpp:107-119).  Budgets that run
+    out inside an output interval give AMICI_TOO_MUCH_WORK; one used up
+    exactly at an output time gives AMICI_ERROR.  Stores per budget the status
+    and the index of the first output time not reached, and the cumulative
+    steps to each output time."""
+    net = networks.robertson()
+    theta = net.theta_nominal[None, :]
+    ts = networks.robertson_times()
+    k0 = np.zeros((1, 0))
+    steps = np.array([O.simulate(net, theta, k0, ts[:k], **TIGHT)["counters"][0, 0]
+                      for k in range(1, len(ts) + 1)])
+    budgets = np.array([1, int(steps[10]) - 1, int(steps[10]), 500, int(steps[20]) + 1])
+    status, first_bad = [], []
+    for m in budgets:
+        r = O.simulate(net, theta, k0, ts, maxsteps=int(m), **TIGHT)
+        status.append(int(r["status"][0]))
+        first_bad.append(int((steps <= m).sum()))
+    np.savez_compressed(OUT / "robertson_step_limit.npz", theta=theta, ts=ts, steps=steps,
+                        budgets=budgets, status=np.array(status), first_bad=np.array(first_bad))
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["n40_counters"]:
         n40_counters()
+    elif sys.argv[1:] == ["step_limit"]:
+        step_limit()
     elif sys.argv[1:] == ["n40_preeq"]:
         n40_preeq()
     else:

@@ -19,6 +19,8 @@ from scipy.linalg import expm
 from amici_amd import codegen, networks
 from oracle import cvodes_oracle as O
 
+GOLD = __import__("pathlib").Path(__file__).resolve().parent / "golden"
+
 
 def _linear_A(net, theta):
     """xdot = A x for a first-order network."""
@@ -267,3 +269,22 @@ def test_preequilibration_gradients():
         lp = O.simulate(net, tp, cs["kfix"][b], cs["ts"], meas, sig, **kw)["llh"][0]
         lm = O.simulate(net, tm, cs["kfix"][b], cs["ts"], meas, sig, **kw)["llh"][0]
         assert f["sllh"][0][j] == pytest.approx((lp - lm) / eps, rel=1e-4, abs=1e-6 * scale)
+
+
+def test_step_budget_counts_over_the_whole_run():
+    """Solver::apply_max_num_steps (src/solver.cpp:107-119) hands CVode the
+    steps left of the whole run's budget: a budget used up inside an output
+    interval ends with CV_TOO_MUCH_WORK (-1), one used up exactly at an output
+    time with the AmiException of the next call, AMICI_ERROR (-99,
+    src/amici.cpp:156-157).  The cumulative steps to each output time come
+    from successful runs on truncated time grids, so which case a budget
+    falls into is known independently of the runs with the budget."""
+    g = np.load(GOLD / "robertson_step_limit.npz")
+    net = networks.robertson()
+    steps = g["steps"]
+    assert np.all(np.diff(steps) > 0)
+    for m, st, bad in zip(g["budgets"], g["status"], g["first_bad"]):
+        r = O.simulate(net, g["theta"], np.zeros((1, 0)), g["ts"], maxsteps=int(m))
+        assert int(r["status"][0]) == st
+        assert st == (-99 if m in steps else -1)
+        assert bad == int((steps <= m).sum())
