@@ -38,10 +38,16 @@ HIP_FLAGS = [
 
 
 def default_team(nx: int) -> int:
-    """One instance per 64-lane wavefront (team = wavefront) for nx <= 64."""
-    if nx > 64:
-        raise ValueError("models with more than 64 states need the workgroup-per-instance engine")
-    return 64
+    """One instance per 64-lane wavefront (team = wavefront) for nx <= 64; one
+    instance per workgroup of 2 or 4 wavefronts (team = workgroup) up to 256
+    states."""
+    if nx <= 64:
+        return 64
+    if nx <= 128:
+        return 128
+    if nx <= 256:
+        return 256
+    raise ValueError("models with more than 256 states are not supported (one lane per state)")
 
 
 def _csr(rows: list, n: int):
@@ -151,12 +157,14 @@ def emit_model_traits(net: ReactionNetwork, team: int | None = None) -> str:
     """The AmxModel traits struct (dimensions, team size, sparsity tables)."""
     t = model_tables(net)
     team = team or default_team(net.nx)
-    if team < net.nx or team > 64:
-        raise ValueError(f"team size {team} does not fit nx={net.nx} (1 lane per state, <= 64)")
+    if team < net.nx or not (team in (128, 256) or (team <= 64 and team & (team - 1) == 0)):
+        raise ValueError(f"team size {team} does not fit nx={net.nx} (1 lane per state; "
+                         "a power of two up to 64, or 128 / 256 for a workgroup team)")
     lines = [
         f"// Generated model module for network '{net.name}' (amici_amd.codegen).\n",
         "// Sparsity tables of xdot = S w, J, J^T v, df/dp and y = C x; see\n",
         "// amici_amd/codegen.py:model_tables for the meaning of every table.\n",
+        f"#define AMX_TEAM_LANES {team if team > 64 else 64}\n",
         '#include "amx_massaction.h"\n\n',
         "struct AmxModel {\n",
         f"  static constexpr int NX = {net.nx}, NR = {net.nr}, NP = {net.np}, NK = {net.nk},"

@@ -23,6 +23,8 @@
 // live in registers.  Every function is inlined into the batch kernels.
 // Not modelled: BDF stability-limit detection (cvBDFStab), hmin/hmax limits.
 #pragma once
+#include <type_traits>
+
 #include "amx_team.h"
 
 namespace amx {
@@ -87,13 +89,15 @@ __device__ __forceinline__ double sel(const double (&a)[N], int i) {
   return r;
 }
 
-// LDS block of one integrator instance.
+// LDS block of one integrator instance.  Models with more than 64 states
+// keep their LU factors in global memory (LUBig) instead.
 template <int NX, int NQ>
 struct BdfLds {
   static constexpr int NQP = NQ > 0 ? NQ : 1;
+  static constexpr int NLU = NX <= 64 ? NX : 1;
   double zn[LMAX][NX];    // Nordsieck history of the states (column i: lane i)
   double znQ[LMAX][NQP];  // Nordsieck history of the quadratures
-  double lu[NX][NX];      // LU factors: (row i, column j) at lu[j][i]
+  double lu[NLU][NLU];    // LU factors: (row i, column j) at lu[j][i]
   double tau[LMAX + 1];   // previous step sizes
   double l[LMAX];         // method coefficients
   double tq[6];           // error-test constants
@@ -179,6 +183,212 @@ struct LURows {
   }
 };
 
+// Blocked row-per-lane dense LU of an NX x NX matrix (64 < NX <= T) for a
+// multi-wavefront team.  The matrix lives in global memory, column-major
+// (row i, column j at A[j * NX + i], so the lanes of a column access are
+// consecutive), and is factorised in place; lane i owns row i.  Partial
+// pivoting as in LURows (largest magnitude, ties to the smaller row, rows
+// never move: perm[k] = row chosen at step k), in panels of NB columns
+// (right-looking): the panel is eliminated in registers with one pivot search
+// and one pivot-row broadcast per column, the pivot rows' trailing entries
+// become U12 = L11^-1 A12 (one lane per column), and the rows not yet chosen
+// take the rank-NB update A22 -= L21 U12 with U12 staged in LDS.  The
+// solves run block by block as well: the NB pivot values of a block are
+// published through LDS, every lane solves the small triangle, and the other
+// rows update with one fused pass over their NB factors.
+template <int NX, int T>
+struct LUBig {
+  static_assert(NX > 64 && NX <= T, "LUBig: 64 < NX <= T");
+  static constexpr int NB = 16;
+  struct Shared {
+    double u[NB][T];      // U12 rows of the panel (column k1 + j at index j)
+    double l11[NB][NB];   // factors of the panel's pivot rows inside the panel
+    double prow[NB];      // pivot-row panel values
+    double y[NB];         // solve: values of the block's pivot rows
+    double d[NB];         // solve: their inverse pivots
+  };
+  static __device__ __forceinline__ Shared& shared() {
+    __shared__ Shared sh;
+    return sh;
+  }
+
+  double dinv;  // 1 / pivot of the row this lane holds
+  int pos;      // elimination step at which this lane's row was the pivot
+  int src;      // lane that holds pivot row number `lane`
+
+  // Factorises A in place; false for a zero or non-finite pivot.
+  template <class L>
+  __device__ __forceinline__ bool factor(const Team<T>& tm, double* A, L& S) {
+    Shared& sh = shared();
+    const int i = tm.lane;
+    const bool act = i < NX;
+    pos = act ? -1 : NX;
+    src = 0;
+    dinv = 0.0;
+    bool ok = true;
+    for (int k0 = 0; k0 < NX && ok; k0 += NB) {
+      const int kb = (NX - k0) < NB ? (NX - k0) : NB;
+      double pa[NB];
+#pragma unroll
+      for (int c = 0; c < NB; ++c) pa[c] = (act && c < kb) ? A[(size_t)(k0 + c) * NX + i] : 0.0;
+#pragma unroll
+      for (int c = 0; c < NB; ++c) {
+        if (c >= kb || !ok) continue;  // team-uniform
+        const int k = k0 + c;
+        double v = (act && pos < 0) ? fabs(pa[c]) : -1.0;
+        if (v != v) v = INFINITY;  // a NaN entry fails the factorisation
+        int p;
+        const double mv = tm.argmax(v, p);
+        if (!(mv > 0.0) || !(mv < INFINITY)) {
+          ok = false;
+          continue;
+        }
+        if (i == 0) S.perm[k] = p;
+        if (i == k) src = p;
+        if (i == p) {
+#pragma unroll
+          for (int j = c; j < NB; ++j) sh.prow[j] = pa[j];
+        }
+        __syncthreads();
+        double pr[NB];
+#pragma unroll
+        for (int j = c; j < NB; ++j) pr[j] = sh.prow[j];
+        __syncthreads();
+        const double inv = 1.0 / pr[c];
+        if (i == p) { pos = k; dinv = inv; }
+        if (act && pos < 0) {
+          const double l = pa[c] * inv;
+          pa[c] = l;
+#pragma unroll
+          for (int j = c + 1; j < NB; ++j) pa[j] = fma(-pr[j], l, pa[j]);
+        }
+      }
+      if (!ok) break;  // team-uniform
+#pragma unroll
+      for (int c = 0; c < NB; ++c)
+        if (act && c < kb) A[(size_t)(k0 + c) * NX + i] = pa[c];
+      const int k1 = k0 + kb;
+      if (k1 >= NX) break;
+      // factors of the panel's pivot rows inside the panel
+      if (act && pos >= k0 && pos < k1) {
+        const int c = pos - k0;
+#pragma unroll
+        for (int cc = 0; cc < NB; ++cc) sh.l11[c][cc] = pa[cc];
+      }
+      __syncthreads();
+      // U12: lane j finishes the pivot rows' entries of trailing column k1 + j
+      const int col = k1 + i;
+      if (col < NX) {
+        double u[NB];
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+          if (c < kb) {
+            const int pr = S.perm[k0 + c];
+            double val = A[(size_t)col * NX + pr];
+#pragma unroll
+            for (int cc = 0; cc < c; ++cc) val = fma(-sh.l11[c][cc], u[cc], val);
+            u[c] = val;
+            A[(size_t)col * NX + pr] = val;
+            sh.u[c][i] = val;
+          } else {
+            u[c] = 0.0;
+            sh.u[c][i] = 0.0;
+          }
+        }
+      }
+      __syncthreads();
+      // A22 -= L21 U12 on the rows not chosen yet
+      if (act && pos < 0) {
+        const int nc = NX - k1;
+        for (int jj = 0; jj < nc; ++jj) {
+          double* e = A + (size_t)(k1 + jj) * NX + i;
+          double val = *e;
+#pragma unroll
+          for (int c = 0; c < NB; ++c) val = fma(-pa[c], sh.u[c][jj], val);
+          *e = val;
+        }
+      }
+      __syncthreads();
+    }
+    return ok;
+  }
+
+  // Solves (LU) x = b with the factors in A; b is this lane's component.
+  template <class L>
+  __device__ __forceinline__ double solve(double b, const Team<T>& tm, const double* A,
+                                          const L& S) const {
+    Shared& sh = shared();
+    const int i = tm.lane;
+    const bool act = i < NX;
+    // forward substitution L y = P b, NB pivots at a time
+    for (int k0 = 0; k0 < NX; k0 += NB) {
+      const int kb = (NX - k0) < NB ? (NX - k0) : NB;
+      if (act && pos >= k0 && pos < k0 + kb) {
+        const int c = pos - k0;
+        sh.y[c] = b;
+        for (int cc = 0; cc < c; ++cc) sh.l11[c][cc] = A[(size_t)(k0 + cc) * NX + i];
+      }
+      __syncthreads();
+      double y[NB];
+#pragma unroll
+      for (int c = 0; c < NB; ++c) {
+        double v = 0.0;
+        if (c < kb) {
+          v = sh.y[c];
+#pragma unroll
+          for (int cc = 0; cc < c; ++cc) v = fma(-sh.l11[c][cc], y[cc], v);
+        }
+        y[c] = v;
+      }
+      __syncthreads();
+      if (act) {
+        if (pos >= k0 && pos < k0 + kb) {
+          b = sel(y, pos - k0);
+        } else if (pos >= k0 + kb) {
+#pragma unroll
+          for (int c = 0; c < NB; ++c)
+            if (c < kb) b = fma(-A[(size_t)(k0 + c) * NX + i], y[c], b);
+        }
+      }
+    }
+    // back substitution U x = y, blocks from the last one
+    for (int k0 = ((NX - 1) / NB) * NB; k0 >= 0; k0 -= NB) {
+      const int kb = (NX - k0) < NB ? (NX - k0) : NB;
+      if (act && pos >= k0 && pos < k0 + kb) {
+        const int c = pos - k0;
+        sh.y[c] = b;
+        sh.d[c] = dinv;
+        for (int cc = c + 1; cc < kb; ++cc) sh.l11[c][cc] = A[(size_t)(k0 + cc) * NX + i];
+      }
+      __syncthreads();
+      double z[NB];
+#pragma unroll
+      for (int c = NB - 1; c >= 0; --c) {
+        double v = 0.0;
+        if (c < kb) {
+          v = sh.y[c];
+#pragma unroll
+          for (int cc = c + 1; cc < NB; ++cc)
+            if (cc < kb) v = fma(-sh.l11[c][cc], z[cc], v);
+          v *= sh.d[c];
+        }
+        z[c] = v;
+      }
+      __syncthreads();
+      if (act) {
+        if (pos >= k0 && pos < k0 + kb) {
+          b = sel(z, pos - k0);
+        } else if (pos < k0) {
+#pragma unroll
+          for (int c = 0; c < NB; ++c)
+            if (c < kb) b = fma(-A[(size_t)(k0 + c) * NX + i], z[c], b);
+        }
+      }
+    }
+    return tm.gather(b, act ? src : 0);
+  }
+};
+
 constexpr int TRACE_REC = 8;
 
 struct SolverParams {
@@ -218,7 +428,10 @@ struct BDF {
   // per-lane work values
   double ewt, acor, y, ftemp;
   double ewtQ[EPQ], acorQ[EPQ], yQ[EPQ];
-  LURows<NX, T> lu;
+  // LU in LDS and registers for one-wavefront teams, in global memory
+  // (the problem's per-instance matrix pb.mat) for multi-wavefront teams
+  static constexpr bool GLU = T > 64;
+  std::conditional_t<GLU, LUBig<NX, T>, LURows<NX, T>> lu;
 
   // team-uniform scalars
   double tn, h, hprime, hscale, eta, etamax, etaq, etaqm1, etaqp1, gamma,
@@ -577,9 +790,15 @@ struct BDF {
     } else {
       jcur = false;
     }
-    double a[NX];
-    pb.build_M(gamma, a);
-    const bool ok = lu.factor(tm, a, S);
+    bool ok;
+    if constexpr (GLU) {
+      pb.build_M_global(gamma);
+      ok = lu.factor(tm, pb.mat, S);
+    } else {
+      double a[NX];
+      pb.build_M(gamma, a);
+      ok = lu.factor(tm, a, S);
+    }
     nsetups++;
     gamrat = 1.0;
     gammap = gamma;
@@ -590,7 +809,9 @@ struct BDF {
   }
 
   __device__ __forceinline__ double lsolve(double b) const {
-    double x = lu.solve(b, tm, S);
+    double x;
+    if constexpr (GLU) x = lu.solve(b, tm, pb.mat, S);
+    else x = lu.solve(b, tm, S);
     if (gamrat != 1.0) x *= 2.0 / (1.0 + gamrat);
     return x;
   }

@@ -48,6 +48,7 @@ struct BatchArgs {
   int newton_maxsteps;     // damped Newton iterations of the pre-equilibration
   double ss_rtol, ss_atol; // steady-state tolerances
   double* xss;             // adjoint: [nb][NX] pre-equilibrated steady state
+  double* lu_ws;           // models with more than 64 states: [nb][NX][NX] Newton matrices
   int* counter;            // work-queue counter (zero before launch)
   unsigned long long maxtime_ticks;  // per-instance wall-clock limit (0 = none)
   double* trace;           // optional step trace of one instance [cap][TRACE_REC]
@@ -61,8 +62,15 @@ struct BatchArgs {
 constexpr int AMX_NSTATS = 16;
 constexpr int SENS_SLOTS = LMAX + 3;  // znS[0..5], acorS, ewtS, sllh partials
 
+// Models with more than 64 states run one instance per workgroup (a
+// multi-wavefront team): their Jacobian is kept as its sparse values Jv in
+// LDS and the Newton matrix in global memory.
+template <class M>
+constexpr bool big_model() { return M::T > 64; }
+
 template <class M>
 struct Smem {
+  static constexpr bool BIG = big_model<M>();
   static constexpr int JLD = M::NX + 1;
   static constexpr int NV = (M::NR > M::NY ? M::NR : M::NY) > M::NX
                                 ? (M::NR > M::NY ? M::NR : M::NY) : M::NX;
@@ -71,7 +79,8 @@ struct Smem {
   double pk[M::NR];   // rate constant of every reaction
   double w[NV];       // fluxes / monomials / observable scratch
   double u[NV];       // S^T v scratch
-  double J[M::NX * JLD];
+  double J[BIG ? 1 : M::NX * JLD];   // dense Jacobian (row-major)
+  double Jv[BIG ? M::JNZ : 1];       // Jacobian values at (J_I[e], J_J[e])
 };
 
 template <class M>
@@ -79,7 +88,8 @@ struct Launch {
   static constexpr int BY_THREADS = 256 / M::T;
   static constexpr int BY_LDS =
       (int)(65536 / (sizeof(Smem<M>) + sizeof(BdfLds<M::NX, M::NP>)));
-  static constexpr int TEAMS = (BY_THREADS < BY_LDS ? BY_THREADS : BY_LDS) > 0
+  static constexpr int TEAMS = big_model<M>() ? 1
+                               : (BY_THREADS < BY_LDS ? BY_THREADS : BY_LDS) > 0
                                    ? (BY_THREADS < BY_LDS ? BY_THREADS : BY_LDS) : 1;
   static constexpr int THREADS = TEAMS * M::T < 64 ? 64 : TEAMS * M::T;
 };
@@ -125,11 +135,14 @@ struct MAEval {
         f = fma(M::SP_C[q], s.w[M::SP_R[q]], f);
     return f;
   }
-  // dense J into s.J (row-major, leading dim JLD); x already in s.x
+  // dense J into s.J (row-major, leading dim JLD), or for big models its
+  // sparse values into s.Jv; x already in s.x
   __device__ __forceinline__ void jacobian() {
     constexpr int JLD = Smem<M>::JLD;
-    for (int e = tm.lane; e < NX * NX; e += T) s.J[(e / NX) * JLD + (e % NX)] = 0.0;
-    team_sync();
+    if constexpr (!Smem<M>::BIG) {
+      for (int e = tm.lane; e < NX * NX; e += T) s.J[(e / NX) * JLD + (e % NX)] = 0.0;
+      team_sync();
+    }
     for (int e = tm.lane; e < M::JNZ; e += T) {
       double v = 0.0;
       for (int t = M::JT_PTR[e]; t < M::JT_PTR[e + 1]; ++t) {
@@ -138,7 +151,24 @@ struct MAEval {
         if (o >= 0) d *= s.x[o];
         v += d;
       }
-      s.J[M::J_I[e] * JLD + M::J_J[e]] = v;
+      if constexpr (Smem<M>::BIG) s.Jv[e] = v;
+      else s.J[M::J_I[e] * JLD + M::J_J[e]] = v;
+    }
+    team_sync();
+  }
+  // Newton matrix I + g * J (TRANS: I + g * J^T) of a big model into the
+  // column-major global matrix A (row i, column j at A[j * NX + i])
+  template <bool TRANS>
+  __device__ __forceinline__ void newton_matrix(double g, double* A) {
+    for (int e = tm.lane; e < NX * NX; e += T) {
+      const int j = e / NX, i = e - j * NX;
+      A[e] = (i == j) ? 1.0 : 0.0;
+    }
+    team_sync();
+    for (int e = tm.lane; e < M::JNZ; e += T) {
+      const int r = TRANS ? M::J_J[e] : M::J_I[e];
+      const int c = TRANS ? M::J_I[e] : M::J_J[e];
+      A[(size_t)c * NX + r] += g * s.Jv[e];
     }
     team_sync();
   }
@@ -197,6 +227,10 @@ struct FwdProblem {
 #pragma unroll
     for (int j = 0; j < NX; ++j)
       a[j] = (i < NX) ? ((i == j ? 1.0 : 0.0) - gamma * s.J[i * JLD + j]) : 0.0;
+  }
+  double* mat = nullptr;  // big models: this instance's Newton matrix (global)
+  __device__ __forceinline__ void build_M_global(double gamma) {
+    ev.template newton_matrix<false>(-gamma, mat);
   }
 
   // ---------- forward sensitivities (global-memory Nordsieck arrays) ----------
@@ -550,6 +584,10 @@ struct BwdProblem {
     for (int j = 0; j < NX; ++j)
       a[j] = (i < NX) ? ((i == j ? 1.0 : 0.0) + gamma * s.J[j * JLD + i]) : 0.0;
   }
+  double* mat = nullptr;  // big models: this instance's Newton matrix (global)
+  __device__ __forceinline__ void build_M_global(double gamma) {
+    ev.template newton_matrix<true>(gamma, mat);
+  }
 };
 
 // ---------------------------------------------------------------------------
@@ -733,7 +771,8 @@ __device__ __forceinline__ void simulate_forward(int inst, const BatchArgs& a, c
   // (reinitialize_fixed_parameter_initial_states)
   int pre_status = AMX_SUCCESS;
   double xss = 0.0;
-  if (a.kfix_pre) {
+  constexpr bool BIG = big_model<M>();
+  if constexpr (!BIG) if (a.kfix_pre) {
     double xp = 0.0;
     if (tm.lane < NX) {
       const int kx = M::X0K[tm.lane];
@@ -757,6 +796,7 @@ __device__ __forceinline__ void simulate_forward(int inst, const BatchArgs& a, c
     pb0.tf = a.traj_f + I * (size_t)a.max_traj * NX;
     pb0.max_traj = a.max_traj;
   }
+  if constexpr (BIG) pb0.mat = a.lu_ws + I * (size_t)NX * NX;
   PB& pb = pb0;
   BDF<PB> bdf(pb, lds, a.sp);
   if (a.maxtime_ticks) bdf.sp.deadline = wall_clock64() + a.maxtime_ticks;
@@ -871,6 +911,8 @@ __device__ __forceinline__ void simulate_backward(int inst, const BatchArgs& a, 
   const double* tt = a.traj_t + I * (size_t)a.max_traj;
   PB pb0(tm, s, tt, a.traj_y + I * (size_t)a.max_traj * NX,
          a.traj_f + I * (size_t)a.max_traj * NX, n);
+  constexpr bool BIG = big_model<M>();
+  if constexpr (BIG) pb0.mat = a.lu_ws + I * (size_t)NX * NX;
   BDF<PB> bdf(pb0, lds, a.spB);  // holds pb0 and the LDS block by reference
   if (a.maxtime_ticks) bdf.sp.deadline = wall_clock64() + a.maxtime_ticks;
   const double* ts = a.ts + (size_t)a.ts_stride * I;
@@ -925,7 +967,7 @@ __device__ __forceinline__ void simulate_backward(int inst, const BatchArgs& a, 
   double gss[EPQ];
 #pragma unroll
   for (int e = 0; e < EPQ; ++e) gss[e] = 0.0;
-  if (a.kfix_pre && status == AMX_SUCCESS) {
+  if constexpr (!BIG) if (a.kfix_pre && status == AMX_SUCCESS) {
     const double xs = a.xss[I * NX + (tm.lane < NX ? tm.lane : 0)];
     const double xbm = (tm.lane < NX && M::X0K[tm.lane] < 0) ? xB : 0.0;
     LURows<NX, T> lu;
@@ -985,6 +1027,10 @@ __device__ __forceinline__ int next_instance(const Team<M::T>& tm, int* counter)
 template <class M, int MODE>
 __device__ __forceinline__ void forward_kernel_body(BatchArgs a) {
   constexpr int T = M::T;
+  // forward sensitivities of models with more than 64 states are not
+  // provided (the C ABI rejects them before launch)
+  if constexpr (MODE == 1 && big_model<M>()) return;
+  else {
   __shared__ Smem<M> smem[Launch<M>::TEAMS];
   __shared__ BdfLds<M::NX, 0> blds[Launch<M>::TEAMS];
   Team<T> tm;
@@ -994,6 +1040,7 @@ __device__ __forceinline__ void forward_kernel_body(BatchArgs a) {
     const int inst = next_instance<M>(tm, a.counter);
     if (inst >= a.nb) break;
     simulate_forward<M, MODE>(inst, a, tm, s, lds);
+  }
   }
 }
 

@@ -1,14 +1,27 @@
-// Team primitives for gfx950: a "team" is T consecutive lanes of one 64-wide
-// wavefront (T a power of two, 1..64) that integrates one ODE instance.  All
-// team-level control flow is uniform inside a team; different teams of the
-// same wavefront may diverge, so no workgroup barrier is ever used here.
+// Team primitives for gfx950: a "team" integrates one ODE instance.  Two
+// kinds:
+//  * T = 1..64 (a power of two): T consecutive lanes of one 64-wide
+//    wavefront.  Different teams of the same wavefront may diverge, so no
+//    workgroup barrier is used.
+//  * T = 128 or 256: the whole workgroup (T / 64 wavefronts, one team per
+//    block), for models with more than 64 states.  Cross-wavefront
+//    reductions and exchanges go through LDS with workgroup barriers.
+// All team-level control flow is uniform inside a team.  A translation unit
+// holds one kind of multi-wavefront team: AMX_TEAM_LANES names its size.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#ifndef AMX_TEAM_LANES
+#define AMX_TEAM_LANES 64
+#endif
+
 namespace amx {
 
 __device__ __forceinline__ void team_sync() {
+#if AMX_TEAM_LANES > 64
+  __syncthreads();
+#else
   // A team never spans more than one wavefront, so wavefront-scope fences
   // order its LDS traffic: they keep the compiler from moving LDS accesses
   // across the point, and LDS requests of one wavefront are serviced in issue
@@ -16,6 +29,7 @@ __device__ __forceinline__ void team_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#endif
 }
 
 // Value of v on wave lane `lane` (lane uniform).  The double is split into
@@ -53,37 +67,119 @@ __device__ __forceinline__ double dpp_d(double v) {
   return split_d(v, [](int x) { return dpp_i<CTRL>(x); });
 }
 
-template <int T>
-struct Team {
-  static_assert(T >= 1 && T <= 64 && (T & (T - 1)) == 0, "team size");
-  int lane;  // lane inside the team, 0..T-1
-  int base;  // wave lane of team lane 0
-
-  __device__ __forceinline__ Team() {
-    const int wl = __lane_id();
-    lane = wl & (T - 1);
-    base = wl - lane;
-  }
-
-  // Reductions.  For a full-wave team (T = 64) each row of 16 lanes reduces
-  // with DPP moves (xor 1, xor 2, half mirror, mirror: every lane of the row
-  // combines the same operand pairs, so the row result is the same on all its
-  // lanes) and the four row results are combined from lanes 0, 16, 32 and 48
-  // with readlane in a fixed order, so the result is one scalar value for the
-  // whole wave (uniform by construction).  Smaller teams use a butterfly of
-  // lane shuffles.
+// Full-wavefront reductions.  Each row of 16 lanes reduces with DPP moves
+// (xor 1, xor 2, half mirror, mirror: every lane of the row combines the same
+// operand pairs, so the row result is the same on all its lanes) and the four
+// row results are combined from lanes 0, 16, 32 and 48 with readlane in a
+// fixed order, so the result is one scalar value for the whole wavefront
+// (uniform by construction).
+struct Wave64 {
   template <class Op>
-  __device__ __forceinline__ double row_reduce(double v, Op op) const {
+  static __device__ __forceinline__ double row_reduce(double v, Op op) {
     v = op(v, dpp_d<DPP_XOR1>(v));
     v = op(v, dpp_d<DPP_XOR2>(v));
     v = op(v, dpp_d<DPP_HALF_MIRROR>(v));
     v = op(v, dpp_d<DPP_MIRROR>(v));
     return v;
   }
+  static __device__ __forceinline__ double sum(double v) {
+    v = row_reduce(v, [](double a, double b) { return a + b; });
+    return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
+  }
+  static __device__ __forceinline__ double max(double v) {
+    v = row_reduce(v, [](double a, double b) { return fmax(a, b); });
+    return fmax(fmax(readlane_d(v, 0), readlane_d(v, 16)),
+                fmax(readlane_d(v, 32), readlane_d(v, 48)));
+  }
+  static __device__ __forceinline__ int maxi(int v) {
+    int u;
+    u = dpp_i<DPP_XOR1>(v); v = u > v ? u : v;
+    u = dpp_i<DPP_XOR2>(v); v = u > v ? u : v;
+    u = dpp_i<DPP_HALF_MIRROR>(v); v = u > v ? u : v;
+    u = dpp_i<DPP_MIRROR>(v); v = u > v ? u : v;
+    int r = __builtin_amdgcn_readlane(v, 0);
+    u = __builtin_amdgcn_readlane(v, 16); r = u > r ? u : r;
+    u = __builtin_amdgcn_readlane(v, 32); r = u > r ? u : r;
+    u = __builtin_amdgcn_readlane(v, 48); r = u > r ? u : r;
+    return r;
+  }
+  // (value, index) maximum; ties resolve to the smaller index.
+  static __device__ __forceinline__ double argmax(double v, int i, int& idx) {
+#define AMX_ARGMAX_STEP(CTRL)                                \
+    {                                                        \
+      const double u = dpp_d<CTRL>(v);                       \
+      const int j = dpp_i<CTRL>(i);                          \
+      if (u > v || (u == v && j < i)) { v = u; i = j; }      \
+    }
+    AMX_ARGMAX_STEP(DPP_XOR1)
+    AMX_ARGMAX_STEP(DPP_XOR2)
+    AMX_ARGMAX_STEP(DPP_HALF_MIRROR)
+    AMX_ARGMAX_STEP(DPP_MIRROR)
+#undef AMX_ARGMAX_STEP
+    double bv = readlane_d(v, 0);
+    int bi = __builtin_amdgcn_readlane(i, 0);
+#pragma unroll
+    for (int row = 16; row < 64; row += 16) {
+      const double rv = readlane_d(v, row);
+      const int ri = __builtin_amdgcn_readlane(i, row);
+      if (rv > bv || (rv == bv && ri < bi)) { bv = rv; bi = ri; }
+    }
+    idx = bi;
+    return bv;
+  }
+};
+
+// LDS exchange area of a multi-wavefront team (one team per workgroup).
+struct TeamXchg {
+  double d[AMX_TEAM_LANES > 64 ? AMX_TEAM_LANES : 1];
+  int i[AMX_TEAM_LANES > 64 ? AMX_TEAM_LANES : 1];
+};
+__device__ __forceinline__ TeamXchg& team_xchg() {
+  __shared__ TeamXchg x;
+  return x;
+}
+
+template <int T>
+struct Team {
+  static_assert((T >= 1 && T <= 64 && (T & (T - 1)) == 0) ||
+                    ((T == 128 || T == 256) && T == AMX_TEAM_LANES),
+                "team size");
+  static constexpr int W = T > 64 ? T / 64 : 1;  // wavefronts per team
+  int lane;  // lane inside the team, 0..T-1
+  int base;  // wave lane of team lane 0 (single-wavefront teams)
+
+  __device__ __forceinline__ Team() {
+    if constexpr (T > 64) {
+      lane = threadIdx.x;
+      base = 0;
+    } else {
+      const int wl = __lane_id();
+      lane = wl & (T - 1);
+      base = wl - lane;
+    }
+  }
+
+  // Multi-wavefront combine: every wavefront's (uniform) partial is written
+  // by its first lane, the team waits, every lane folds the W partials in
+  // wavefront order (the same order on every lane, so the result is team
+  // uniform), and the team waits again before the area can be reused.
+  template <class Op>
+  __device__ __forceinline__ double combine(double wv, Op op) const {
+    TeamXchg& x = team_xchg();
+    if (__lane_id() == 0) x.d[lane >> 6] = wv;
+    __syncthreads();
+    double r = x.d[0];
+#pragma unroll
+    for (int w = 1; w < W; ++w) r = op(r, x.d[w]);
+    __syncthreads();
+    return r;
+  }
+
   __device__ __forceinline__ double sum(double v) const {
-    if constexpr (T == 64) {
-      v = row_reduce(v, [](double a, double b) { return a + b; });
-      return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
+    if constexpr (T > 64) {
+      return combine(Wave64::sum(v), [](double a, double b) { return a + b; });
+    } else if constexpr (T == 64) {
+      return Wave64::sum(v);
     } else {
 #pragma unroll
       for (int o = T / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, T);
@@ -91,10 +187,10 @@ struct Team {
     }
   }
   __device__ __forceinline__ double max(double v) const {
-    if constexpr (T == 64) {
-      v = row_reduce(v, [](double a, double b) { return fmax(a, b); });
-      return fmax(fmax(readlane_d(v, 0), readlane_d(v, 16)),
-                  fmax(readlane_d(v, 32), readlane_d(v, 48)));
+    if constexpr (T > 64) {
+      return combine(Wave64::max(v), [](double a, double b) { return fmax(a, b); });
+    } else if constexpr (T == 64) {
+      return Wave64::max(v);
     } else {
 #pragma unroll
       for (int o = T / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, T));
@@ -102,17 +198,18 @@ struct Team {
     }
   }
   __device__ __forceinline__ int maxi(int v) const {
-    if constexpr (T == 64) {
-      int u;
-      u = dpp_i<DPP_XOR1>(v); v = u > v ? u : v;
-      u = dpp_i<DPP_XOR2>(v); v = u > v ? u : v;
-      u = dpp_i<DPP_HALF_MIRROR>(v); v = u > v ? u : v;
-      u = dpp_i<DPP_MIRROR>(v); v = u > v ? u : v;
-      int r = __builtin_amdgcn_readlane(v, 0);
-      u = __builtin_amdgcn_readlane(v, 16); r = u > r ? u : r;
-      u = __builtin_amdgcn_readlane(v, 32); r = u > r ? u : r;
-      u = __builtin_amdgcn_readlane(v, 48); r = u > r ? u : r;
+    if constexpr (T > 64) {
+      const int wv = Wave64::maxi(v);
+      TeamXchg& x = team_xchg();
+      if (__lane_id() == 0) x.i[lane >> 6] = wv;
+      __syncthreads();
+      int r = x.i[0];
+#pragma unroll
+      for (int w = 1; w < W; ++w) r = x.i[w] > r ? x.i[w] : r;
+      __syncthreads();
       return r;
+    } else if constexpr (T == 64) {
+      return Wave64::maxi(v);
     } else {
 #pragma unroll
       for (int o = T / 2; o > 0; o >>= 1) {
@@ -124,30 +221,30 @@ struct Team {
   }
   // (value, index) maximum; ties resolve to the smaller index.
   __device__ __forceinline__ double argmax(double v, int& idx) const {
-    int i = lane;
-    if constexpr (T == 64) {
-#define AMX_ARGMAX_STEP(CTRL)                                \
-      {                                                      \
-        const double u = dpp_d<CTRL>(v);                     \
-        const int j = dpp_i<CTRL>(i);                        \
-        if (u > v || (u == v && j < i)) { v = u; i = j; }    \
+    if constexpr (T > 64) {
+      int wi;
+      const double wv = Wave64::argmax(v, lane, wi);
+      TeamXchg& x = team_xchg();
+      if (__lane_id() == 0) {
+        x.d[lane >> 6] = wv;
+        x.i[lane >> 6] = wi;
       }
-      AMX_ARGMAX_STEP(DPP_XOR1)
-      AMX_ARGMAX_STEP(DPP_XOR2)
-      AMX_ARGMAX_STEP(DPP_HALF_MIRROR)
-      AMX_ARGMAX_STEP(DPP_MIRROR)
-#undef AMX_ARGMAX_STEP
-      double bv = readlane_d(v, 0);
-      int bi = __builtin_amdgcn_readlane(i, 0);
+      __syncthreads();
+      double bv = x.d[0];
+      int bi = x.i[0];
 #pragma unroll
-      for (int row = 16; row < 64; row += 16) {
-        const double rv = readlane_d(v, row);
-        const int ri = __builtin_amdgcn_readlane(i, row);
+      for (int w = 1; w < W; ++w) {
+        const double rv = x.d[w];
+        const int ri = x.i[w];
         if (rv > bv || (rv == bv && ri < bi)) { bv = rv; bi = ri; }
       }
+      __syncthreads();
       idx = bi;
       return bv;
+    } else if constexpr (T == 64) {
+      return Wave64::argmax(v, lane, idx);
     } else {
+      int i = lane;
 #pragma unroll
       for (int o = T / 2; o > 0; o >>= 1) {
         double u = __shfl_xor(v, o, T);
@@ -160,16 +257,45 @@ struct Team {
   }
   // Value of v on team lane src (src uniform within the team).
   __device__ __forceinline__ double bcast(double v, int src) const {
-    if constexpr (T == 64) return readlane_d(v, src);
-    else return __shfl(v, src, T);
+    if constexpr (T > 64) {
+      TeamXchg& x = team_xchg();
+      if (lane == src) x.d[0] = v;
+      __syncthreads();
+      const double r = x.d[0];
+      __syncthreads();
+      return r;
+    } else if constexpr (T == 64) {
+      return readlane_d(v, src);
+    } else {
+      return __shfl(v, src, T);
+    }
   }
   __device__ __forceinline__ int bcast_i(int v, int src) const {
-    if constexpr (T == 64) return __builtin_amdgcn_readlane(v, src);
-    else return __shfl(v, src, T);
+    if constexpr (T > 64) {
+      TeamXchg& x = team_xchg();
+      if (lane == src) x.i[0] = v;
+      __syncthreads();
+      const int r = x.i[0];
+      __syncthreads();
+      return r;
+    } else if constexpr (T == 64) {
+      return __builtin_amdgcn_readlane(v, src);
+    } else {
+      return __shfl(v, src, T);
+    }
   }
-  // Value of v on team lane src (src may differ per lane).
+  // Value of v on team lane src (src may differ per lane, 0 <= src < T).
   __device__ __forceinline__ double gather(double v, int src) const {
-    return __shfl(v, src, T);
+    if constexpr (T > 64) {
+      TeamXchg& x = team_xchg();
+      x.d[lane] = v;
+      __syncthreads();
+      const double r = x.d[src];
+      __syncthreads();
+      return r;
+    } else {
+      return __shfl(v, src, T);
+    }
   }
 };
 

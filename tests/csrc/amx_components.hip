@@ -1,5 +1,6 @@
 // Component kernels for the GPU tests: each one exercises a single building
-// block of the integrator (team reductions / broadcasts, row-per-lane LU,
+// block of the integrator (team reductions / broadcasts, row-per-lane LU or
+// the blocked global-memory LU of models with more than 64 states,
 // mass-action model evaluation) on inputs from the host, with no integration
 // loop, so results can be compared element by element with numpy.
 // Compiled together with a generated model's traits (AMX_MODEL_TRAITS).
@@ -29,22 +30,49 @@ __global__ void __launch_bounds__(THREADS) k_team(const double* in, double* out,
   o[0] = s; o[1] = mx; o[2] = am; o[3] = idx; o[4] = b; o[5] = g;
 }
 
-// LU factor of A (row i on lane i) and solve A x = b
+// LU factor of A (row-major input, row i on lane i) and solve A x = b; models
+// with more than 64 states factorise a column-major copy in `work` (LUBig)
+template <class MM, bool BIG = amx::big_model<MM>()>
+struct LuCase {  // row-per-lane LU in registers and LDS
+  static __device__ __forceinline__ bool run(const amx::Team<T>& tm, const double* A, double bi,
+                                             double* xi, double*, amx::BdfLds<NX, 0>& S) {
+    const int i = tm.lane;
+    amx::LURows<NX, T> lu;
+    double a[NX];
+#pragma unroll
+    for (int j = 0; j < NX; ++j) a[j] = (i < NX) ? A[(size_t)i * NX + j] : 0.0;
+    const bool good = lu.factor(tm, a, S);
+    *xi = lu.solve(bi, tm, S);
+    return good;
+  }
+};
+template <class MM>
+struct LuCase<MM, true> {  // blocked LU in global memory
+  static __device__ __forceinline__ bool run(const amx::Team<T>& tm, const double* A, double bi,
+                                             double* xi, double* W, amx::BdfLds<NX, 0>& S) {
+    const int i = tm.lane;
+    amx::LUBig<MM::NX, MM::T> lu;
+    if (i < NX)
+      for (int j = 0; j < NX; ++j) W[(size_t)j * NX + i] = A[(size_t)i * NX + j];
+    amx::team_sync();
+    const bool good = lu.factor(tm, W, S);
+    *xi = lu.solve(bi, tm, W, S);
+    return good;
+  }
+};
+
 __global__ void __launch_bounds__(THREADS) k_lu(const double* A, const double* b, double* x, int* ok,
-                                            int nteams) {
+                                            double* work, int nteams) {
   amx::Team<T> tm;
   const int team = (blockIdx.x * blockDim.x + threadIdx.x) / T;
   if (team >= nteams) return;
   __shared__ amx::BdfLds<NX, 0> lds[TEAMS];
-  amx::LURows<NX, T> lu;
-  const int i = tm.lane;
-  double a[NX];
-#pragma unroll
-  for (int j = 0; j < NX; ++j) a[j] = (i < NX) ? A[((size_t)team * NX + i) * NX + j] : 0.0;
   amx::BdfLds<NX, 0>& S = lds[threadIdx.x / T];
-  const bool good = lu.factor(tm, a, S);
+  const int i = tm.lane;
   const double bi = (i < NX) ? b[(size_t)team * NX + i] : 0.0;
-  const double xi = lu.solve(bi, tm, S);
+  double xi = 0.0;
+  const bool good = LuCase<M>::run(tm, A + (size_t)team * NX * NX, bi, &xi,
+                                                     work + (size_t)team * NX * NX, S);
   if (i < NX) x[(size_t)team * NX + i] = good ? xi : NAN;
   if (i == 0) ok[team] = good ? 1 : 0;
 }
@@ -66,8 +94,15 @@ __global__ void __launch_bounds__(THREADS) k_model(const double* theta, const do
   if (tm.lane < NX) xdot[(size_t)team * NX + tm.lane] = f;
   ev.put_x(xi);
   ev.jacobian();
-  for (int e = tm.lane; e < NX * NX; e += T)
-    J[(size_t)team * NX * NX + e] = s.J[(e / NX) * amx::Smem<M>::JLD + (e % NX)];
+  if constexpr (amx::big_model<M>()) {
+    for (int e = tm.lane; e < NX * NX; e += T) J[(size_t)team * NX * NX + e] = 0.0;
+    amx::team_sync();
+    for (int e = tm.lane; e < M::JNZ; e += T)
+      J[(size_t)team * NX * NX + M::J_I[e] * NX + M::J_J[e]] = s.Jv[e];
+  } else {
+    for (int e = tm.lane; e < NX * NX; e += T)
+      J[(size_t)team * NX * NX + e] = s.J[(e / NX) * amx::Smem<M>::JLD + (e % NX)];
+  }
   amx::team_sync();
   // adjoint RHS through the flux structure, as BwdProblem::rhs
   if (tm.lane < NX) s.v[tm.lane] = v[(size_t)team * NX + tm.lane];
@@ -98,9 +133,10 @@ int amxt_team(const double* in, double* out, int nteams, void* stream) {
                      nteams);
   return hipGetLastError() == hipSuccess ? 0 : -30;
 }
-int amxt_lu(const double* A, const double* b, double* x, int* ok, int nteams, void* stream) {
+int amxt_lu(const double* A, const double* b, double* x, int* ok, double* work, int nteams,
+            void* stream) {
   hipLaunchKernelGGL(k_lu, dim3(blocks_for(nteams)), dim3(THREADS), 0, (hipStream_t)stream, A, b, x,
-                     ok, nteams);
+                     ok, work, nteams);
   return hipGetLastError() == hipSuccess ? 0 : -30;
 }
 int amxt_model(const double* theta, const double* x, const double* v, double* xdot, double* J,

@@ -148,6 +148,20 @@ def n40_counters() -> None:
                         llh=r["llh"])
 
 
+def n200() -> None:
+    """4 conditions of the 200-species network: states, observables, llh and
+    the adjoint gradient at rtol 1e-10 / atol 1e-14, plus the rtol-1e-8
+    counters of the adjoint evaluation (the algorithmic work per condition)."""
+    cs, theta, kfix, meas, sigma = measurements("N200", 4)
+    net, ts = cs["network"], cs["ts"]
+    a = run(net, theta, kfix, ts, meas, sigma, 2, TIGHT)
+    c = run(net, theta, kfix, ts, meas, sigma, 2, BENCH)
+    np.savez_compressed(OUT / "n200_conditions4.npz", theta=theta, kfix=kfix, ts=ts, meas=meas,
+                        sigma=sigma, x=a["x"], y=a["y"], llh=a["llh"], sllh_asa=a["sllh"],
+                        counters_asa=a["counters"].astype(np.int32),
+                        counters_bench=c["counters"].astype(np.int32))
+
+
 def step_limit() -> None:
     """Robertson at the nominal parameters with a forward step budget
     (Solver::set_max_steps): the budget counts over the whole run
@@ -177,6 +191,8 @@ if __name__ == "__main__":
         n40_counters()
     elif sys.argv[1:] == ["step_limit"]:
         step_limit()
+    elif sys.argv[1:] == ["n200"]:
+        n200()
     elif sys.argv[1:] == ["n40_preeq"]:
         n40_preeq()
     else:

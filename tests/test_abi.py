@@ -21,11 +21,11 @@ def _declared():
     return sorted(set(re.findall(r"^\s*(?:int|size_t|const char\*)\s+(amx_\w+)\s*\(", text, re.M)))
 
 
-@pytest.fixture(scope="module", params=["robertson", "n40", "linear3", "n40open"])
+@pytest.fixture(scope="module", params=["robertson", "n40", "linear3", "n40open", "n200"])
 def lib(request):
     net = {"robertson": networks.robertson, "n40": networks.network_n40,
            "linear3": lambda: networks.linear_chain(3),
-           "n40open": networks.network_n40_open}[request.param]()
+           "n40open": networks.network_n40_open, "n200": networks.network_n200}[request.param]()
     so = codegen.compile_model(net)
     return net, _lib.ModelLib(so)
 
@@ -39,7 +39,7 @@ def test_exports_and_dims(lib):
     for s in _declared():
         assert hasattr(L.lib, s), s
     assert (L.nx, L.nr, L.np, L.nk, L.ny) == (net.nx, net.nr, net.np, net.nk, net.ny)
-    assert L.team == 64 and L.noise == int(net.noise)
+    assert L.team == (64 if net.nx <= 64 else 256) and L.noise == int(net.noise)
     assert L.status_string(-1) == "AMICI_TOO_MUCH_WORK"
     assert L.status_string(-1001) == "AMICI_NOT_RUN"
 
@@ -55,7 +55,9 @@ def test_workspace_bytes(lib):
     assert L.workspace_bytes(o, 8, 10) < b8
     f = _lib.make_opts(sensi_meth=_lib.SENSI_FORWARD)
     assert L.workspace_bytes(f, 4, 10) >= 4 * net.np * net.nx * 8
-    assert L.workspace_bytes(_lib.make_opts(), 4, 10) <= 512
+    # states only: counters, plus the Newton matrices of a workgroup-team model
+    lu = 4 * net.nx * net.nx * 8 if net.nx > 64 else 0
+    assert lu <= L.workspace_bytes(_lib.make_opts(), 4, 10) <= 512 + lu
 
 
 def test_invalid_arguments_rejected_on_host(lib):
@@ -73,7 +75,11 @@ def test_invalid_arguments_rejected_on_host(lib):
     p = ctypes.cast(buf, ctypes.c_void_p).value
     ins = _lib.BatchIn(1, 3, 0, p, p if net.nk else None, p, None, p, None)
     outs = _lib.BatchOut(None, None, None, p, p, p, None)
-    assert fn(ctypes.byref(o), ctypes.byref(ins), ctypes.byref(outs), p, 8, None) == -23
+    st = _lib.make_opts()
+    assert fn(ctypes.byref(st), ctypes.byref(ins), ctypes.byref(outs), p, 8, None) == -23
+    if net.nx > 64:
+        # workgroup-team models: states and adjoint gradients only
+        assert fn(ctypes.byref(o), ctypes.byref(ins), ctypes.byref(outs), p, 1 << 30, None) == -22
     bad = _lib.make_opts(rtol=-1.0)
     assert fn(ctypes.byref(bad), ctypes.byref(ins), ctypes.byref(outs), p, 8, None) == -22
     bad = _lib.make_opts(rtol=math.nan)

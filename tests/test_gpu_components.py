@@ -1,7 +1,9 @@
 """GPU unit tests of the integrator's building blocks (team reductions and
 broadcasts, row-per-lane LU, mass-action model evaluation) against numpy and
-the oracle's model functions, for a 4-lane team (Robertson, 3 states) and a
-64-lane team (40-species network)."""
+the oracle's model functions, for a 64-lane team with 3 states (Robertson), a
+64-lane team with 40 states (40-species network) and a 256-lane workgroup team
+with 200 states (200-species network: cross-wavefront reductions through LDS,
+blocked LU in global memory)."""
 import ctypes
 
 import numpy as np
@@ -12,7 +14,8 @@ import component_build
 from amici_amd import networks
 from oracle import cvodes_oracle as O
 
-NETS = {"robertson": networks.robertson, "n40": networks.network_n40}
+NETS = {"robertson": networks.robertson, "n40": networks.network_n40,
+        "n200": networks.network_n200}
 
 
 def _lib(name):
@@ -61,7 +64,8 @@ def test_lu_solve(gpu, name):
     At, bt = torch.tensor(A, device=gpu), torch.tensor(b, device=gpu)
     x = torch.zeros((nteams, nx), dtype=torch.float64, device=gpu)
     ok = torch.zeros(nteams, dtype=torch.int32, device=gpu)
-    assert lib.amxt_lu(_p(At), _p(bt), _p(x), _p(ok), nteams, None) == 0
+    work = torch.zeros((nteams, nx, nx), dtype=torch.float64, device=gpu)
+    assert lib.amxt_lu(_p(At), _p(bt), _p(x), _p(ok), _p(work), nteams, None) == 0
     torch.cuda.synchronize()
     assert ok.cpu().numpy().all()
     xr = np.linalg.solve(A, b[..., None])[..., 0]

@@ -183,3 +183,38 @@ def test_preequilibration(open_model, meth):
     if meth:
         key = "sllh_fsa" if meth == 1 else "sllh_asa"
         _close(out["sllh"], g[key], 1e-5, 1e-7, f"sllh ({meth})")
+
+
+@pytest.fixture(scope="module")
+def n200_model(gpu):
+    from amici_amd import sim
+    return sim.Model(networks.network_n200())
+
+
+def test_n200_states(n200_model):
+    """200-species network (one instance per 256-lane workgroup, blocked LU in
+    global memory): states, observables and llh of 4 conditions."""
+    g = _load("n200_conditions4")
+    out = _run({"m": n200_model}, "m", g, 0)
+    _dump("states_n200", out)
+    assert (out["status"] == 0).all(), out["status"]
+    _close(out["x"], g["x"], Y_RTOL, Y_FLOOR, "x")
+    _close(out["y"], g["y"], Y_RTOL, Y_FLOOR, "y")
+    _check_llh(n200_model.network, out, g)
+
+
+def test_n200_adjoint(n200_model):
+    from amici_amd import sim
+    g = _load("n200_conditions4")
+    out = _run({"m": n200_model}, "m", g, sim.SensitivityMethod.adjoint)
+    _dump("asa_n200", out)
+    assert (out["status"] == 0).all(), out["status"]
+    _check_llh(n200_model.network, out, g)
+    _close(out["sllh"], g["sllh_asa"], 1e-5, 1e-7, "sllh (adjoint)")
+
+
+def test_n200_rejects_forward_sensitivities(n200_model):
+    from amici_amd import sim
+    g = _load("n200_conditions4")
+    with pytest.raises(RuntimeError, match="-22"):
+        _run({"m": n200_model}, "m", g, sim.SensitivityMethod.forward)
