@@ -189,23 +189,30 @@ struct LURows {
 // consecutive), and is factorised in place; lane i owns row i.  Partial
 // pivoting as in LURows (largest magnitude, ties to the smaller row, rows
 // never move: perm[k] = row chosen at step k), in panels of NB columns
-// (right-looking): the panel is eliminated in registers with one pivot search
-// and one pivot-row broadcast per column, the pivot rows' trailing entries
-// become U12 = L11^-1 A12 (one lane per column), and the rows not yet chosen
-// take the rank-NB update A22 -= L21 U12 with U12 staged in LDS.  The
-// solves run block by block as well: the NB pivot values of a block are
-// published through LDS, every lane solves the small triangle, and the other
-// rows update with one fused pass over their NB factors.
+// (right-looking): the panel is eliminated in registers; per column, each
+// wavefront's best candidate row publishes its magnitude, index and panel
+// values through LDS in one exchange (double-buffered, one barrier).  The
+// pivot rows' trailing entries become U12 = L11^-1 A12 (one lane per
+// column), and the rows not yet chosen take the rank-NB update
+// A22 -= L21 U12 with U12 staged in LDS.  Global loads are issued in
+// batches ahead of the arithmetic and the stores, so one memory latency
+// covers a batch.  The solves run block by block: the NB pivot values of a
+// block are published through LDS (double-buffered, one barrier per block),
+// every lane solves the small triangle, and the other rows update with
+// their NB factors of the block.
 template <int NX, int T>
 struct LUBig {
   static_assert(NX > 64 && NX <= T, "LUBig: 64 < NX <= T");
-  static constexpr int NB = 16;
+  static constexpr int NB = 16;  // panel width / solve block
+  static constexpr int CB = 8;   // trailing columns per batch of loads
+  static constexpr int W = T / 64;
   struct Shared {
-    double u[NB][T];      // U12 rows of the panel (column k1 + j at index j)
-    double l11[NB][NB];   // factors of the panel's pivot rows inside the panel
-    double prow[NB];      // pivot-row panel values
-    double y[NB];         // solve: values of the block's pivot rows
-    double d[NB];         // solve: their inverse pivots
+    double u[NB][T + CB];       // U12 rows of the panel (column k1 + j at index j)
+    double l11[2][NB][NB];      // factors of a panel's / block's pivot rows inside it
+    double cand[2][W][NB + 1];  // per wavefront: best |value|, then that row's panel values
+    int cidx[2][W];             // per wavefront: row of the best candidate
+    double y[2][NB];            // solve: values of the block's pivot rows
+    double d[2][NB];            // solve: their inverse pivots
   };
   static __device__ __forceinline__ Shared& shared() {
     __shared__ Shared sh;
@@ -221,6 +228,7 @@ struct LUBig {
   __device__ __forceinline__ bool factor(const Team<T>& tm, double* A, L& S) {
     Shared& sh = shared();
     const int i = tm.lane;
+    const int wave = i >> 6;
     const bool act = i < NX;
     pos = act ? -1 : NX;
     src = 0;
@@ -235,25 +243,36 @@ struct LUBig {
       for (int c = 0; c < NB; ++c) {
         if (c >= kb || !ok) continue;  // team-uniform
         const int k = k0 + c;
+        const int buf = k & 1;
         double v = (act && pos < 0) ? fabs(pa[c]) : -1.0;
         if (v != v) v = INFINITY;  // a NaN entry fails the factorisation
-        int p;
-        const double mv = tm.argmax(v, p);
+        int wi;
+        const double wv = Wave64::argmax(v, i, wi);
+        if (i == wi) {
+          sh.cand[buf][wave][0] = wv;
+          sh.cidx[buf][wave] = wi;
+#pragma unroll
+          for (int j = c; j < NB; ++j) sh.cand[buf][wave][1 + j] = pa[j];
+        }
+        lds_barrier();
+        // wavefronts in order, ties to the smaller row: team-uniform winner
+        double mv = sh.cand[buf][0][0];
+        int p = sh.cidx[buf][0], pw = 0;
+#pragma unroll
+        for (int w = 1; w < W; ++w) {
+          const double rv = sh.cand[buf][w][0];
+          const int ri = sh.cidx[buf][w];
+          if (rv > mv || (rv == mv && ri < p)) { mv = rv; p = ri; pw = w; }
+        }
         if (!(mv > 0.0) || !(mv < INFINITY)) {
           ok = false;
           continue;
         }
-        if (i == 0) S.perm[k] = p;
-        if (i == k) src = p;
-        if (i == p) {
-#pragma unroll
-          for (int j = c; j < NB; ++j) sh.prow[j] = pa[j];
-        }
-        __syncthreads();
         double pr[NB];
 #pragma unroll
-        for (int j = c; j < NB; ++j) pr[j] = sh.prow[j];
-        __syncthreads();
+        for (int j = c; j < NB; ++j) pr[j] = sh.cand[buf][pw][1 + j];
+        if (i == 0) S.perm[k] = p;
+        if (i == k) src = p;
         const double inv = 1.0 / pr[c];
         if (i == p) { pos = k; dinv = inv; }
         if (act && pos < 0) {
@@ -273,116 +292,144 @@ struct LUBig {
       if (act && pos >= k0 && pos < k1) {
         const int c = pos - k0;
 #pragma unroll
-        for (int cc = 0; cc < NB; ++cc) sh.l11[c][cc] = pa[cc];
+        for (int cc = 0; cc < NB; ++cc) sh.l11[0][c][cc] = pa[cc];
       }
-      __syncthreads();
+      __syncthreads();  // the panel's rows in global memory are read by other lanes next
       // U12: lane j finishes the pivot rows' entries of trailing column k1 + j
       const int col = k1 + i;
       if (col < NX) {
+        int prr[NB];
         double u[NB];
 #pragma unroll
-        for (int c = 0; c < NB; ++c) {
-          if (c < kb) {
-            const int pr = S.perm[k0 + c];
-            double val = A[(size_t)col * NX + pr];
+        for (int c = 0; c < NB; ++c) prr[c] = (c < kb) ? S.perm[k0 + c] : 0;
 #pragma unroll
-            for (int cc = 0; cc < c; ++cc) val = fma(-sh.l11[c][cc], u[cc], val);
-            u[c] = val;
-            A[(size_t)col * NX + pr] = val;
-            sh.u[c][i] = val;
-          } else {
-            u[c] = 0.0;
-            sh.u[c][i] = 0.0;
+        for (int c = 0; c < NB; ++c) u[c] = (c < kb) ? A[(size_t)col * NX + prr[c]] : 0.0;
+#pragma unroll
+        for (int c = 0; c < NB; ++c)
+          if (c < kb) {
+#pragma unroll
+            for (int cc = 0; cc < c; ++cc) u[c] = fma(-sh.l11[0][c][cc], u[cc], u[c]);
           }
-        }
+#pragma unroll
+        for (int c = 0; c < NB; ++c)
+          if (c < kb) A[(size_t)col * NX + prr[c]] = u[c];
+#pragma unroll
+        for (int c = 0; c < NB; ++c) sh.u[c][i] = u[c];
       }
-      __syncthreads();
-      // A22 -= L21 U12 on the rows not chosen yet
+      lds_barrier();  // U12 rows in LDS; the rows updated next are the lanes' own
+      // A22 -= L21 U12 on the rows not chosen yet, CB columns per batch
       if (act && pos < 0) {
         const int nc = NX - k1;
-        for (int jj = 0; jj < nc; ++jj) {
-          double* e = A + (size_t)(k1 + jj) * NX + i;
-          double val = *e;
+        for (int j0 = 0; j0 < nc; j0 += CB) {
+          double v[CB];
 #pragma unroll
-          for (int c = 0; c < NB; ++c) val = fma(-pa[c], sh.u[c][jj], val);
-          *e = val;
+          for (int q = 0; q < CB; ++q)
+            v[q] = (j0 + q < nc) ? A[(size_t)(k1 + j0 + q) * NX + i] : 0.0;
+#pragma unroll
+          for (int c = 0; c < NB; ++c) {
+            const double l = pa[c];
+#pragma unroll
+            for (int q = 0; q < CB; ++q) v[q] = fma(-l, sh.u[c][j0 + q], v[q]);
+          }
+#pragma unroll
+          for (int q = 0; q < CB; ++q)
+            if (j0 + q < nc) A[(size_t)(k1 + j0 + q) * NX + i] = v[q];
         }
       }
-      __syncthreads();
+      lds_barrier();
     }
+    // U12 entries were written by other lanes than the rows' owners, which
+    // read them in the solves
+    __syncthreads();
     return ok;
   }
 
   // Solves (LU) x = b with the factors in A; b is this lane's component.
+  // Every lane reads only its own row of A: the NB factors of a block are
+  // loaded together at the start of the block.
   template <class L>
   __device__ __forceinline__ double solve(double b, const Team<T>& tm, const double* A,
                                           const L& S) const {
     Shared& sh = shared();
     const int i = tm.lane;
     const bool act = i < NX;
+    constexpr int NBLK = (NX + NB - 1) / NB;
+    int blk = 0;  // running block count: LDS buffer parity
     // forward substitution L y = P b, NB pivots at a time
-    for (int k0 = 0; k0 < NX; k0 += NB) {
+    for (int kb0 = 0; kb0 < NBLK; ++kb0, ++blk) {
+      const int k0 = kb0 * NB;
       const int kb = (NX - k0) < NB ? (NX - k0) : NB;
-      if (act && pos >= k0 && pos < k0 + kb) {
+      const int buf = blk & 1;
+      const bool piv = act && pos >= k0 && pos < k0 + kb;
+      const bool upd = act && pos >= k0 + kb;
+      double f[NB];
+#pragma unroll
+      for (int c = 0; c < NB; ++c) f[c] = ((piv || upd) && c < kb) ? A[(size_t)(k0 + c) * NX + i] : 0.0;
+      if (piv) {
         const int c = pos - k0;
-        sh.y[c] = b;
-        for (int cc = 0; cc < c; ++cc) sh.l11[c][cc] = A[(size_t)(k0 + cc) * NX + i];
+        sh.y[buf][c] = b;
+#pragma unroll
+        for (int cc = 0; cc < NB; ++cc)
+          if (cc < c) sh.l11[buf][c][cc] = f[cc];
       }
-      __syncthreads();
+      lds_barrier();
       double y[NB];
 #pragma unroll
       for (int c = 0; c < NB; ++c) {
         double v = 0.0;
         if (c < kb) {
-          v = sh.y[c];
+          v = sh.y[buf][c];
 #pragma unroll
-          for (int cc = 0; cc < c; ++cc) v = fma(-sh.l11[c][cc], y[cc], v);
+          for (int cc = 0; cc < c; ++cc) v = fma(-sh.l11[buf][c][cc], y[cc], v);
         }
         y[c] = v;
       }
-      __syncthreads();
-      if (act) {
-        if (pos >= k0 && pos < k0 + kb) {
-          b = sel(y, pos - k0);
-        } else if (pos >= k0 + kb) {
+      if (piv) {
+        b = sel(y, pos - k0);
+      } else if (upd) {
 #pragma unroll
-          for (int c = 0; c < NB; ++c)
-            if (c < kb) b = fma(-A[(size_t)(k0 + c) * NX + i], y[c], b);
-        }
+        for (int c = 0; c < NB; ++c)
+          if (c < kb) b = fma(-f[c], y[c], b);
       }
     }
     // back substitution U x = y, blocks from the last one
-    for (int k0 = ((NX - 1) / NB) * NB; k0 >= 0; k0 -= NB) {
+    for (int kb0 = NBLK - 1; kb0 >= 0; --kb0, ++blk) {
+      const int k0 = kb0 * NB;
       const int kb = (NX - k0) < NB ? (NX - k0) : NB;
-      if (act && pos >= k0 && pos < k0 + kb) {
+      const int buf = blk & 1;
+      const bool piv = act && pos >= k0 && pos < k0 + kb;
+      const bool upd = act && pos < k0;
+      double f[NB];
+#pragma unroll
+      for (int c = 0; c < NB; ++c) f[c] = ((piv || upd) && c < kb) ? A[(size_t)(k0 + c) * NX + i] : 0.0;
+      if (piv) {
         const int c = pos - k0;
-        sh.y[c] = b;
-        sh.d[c] = dinv;
-        for (int cc = c + 1; cc < kb; ++cc) sh.l11[c][cc] = A[(size_t)(k0 + cc) * NX + i];
+        sh.y[buf][c] = b;
+        sh.d[buf][c] = dinv;
+#pragma unroll
+        for (int cc = 0; cc < NB; ++cc)
+          if (cc > c && cc < kb) sh.l11[buf][c][cc] = f[cc];
       }
-      __syncthreads();
+      lds_barrier();
       double z[NB];
 #pragma unroll
       for (int c = NB - 1; c >= 0; --c) {
         double v = 0.0;
         if (c < kb) {
-          v = sh.y[c];
+          v = sh.y[buf][c];
 #pragma unroll
           for (int cc = c + 1; cc < NB; ++cc)
-            if (cc < kb) v = fma(-sh.l11[c][cc], z[cc], v);
-          v *= sh.d[c];
+            if (cc < kb) v = fma(-sh.l11[buf][c][cc], z[cc], v);
+          v *= sh.d[buf][c];
         }
         z[c] = v;
       }
-      __syncthreads();
-      if (act) {
-        if (pos >= k0 && pos < k0 + kb) {
-          b = sel(z, pos - k0);
-        } else if (pos < k0) {
+      if (piv) {
+        b = sel(z, pos - k0);
+      } else if (upd) {
 #pragma unroll
-          for (int c = 0; c < NB; ++c)
-            if (c < kb) b = fma(-A[(size_t)(k0 + c) * NX + i], z[c], b);
-        }
+        for (int c = 0; c < NB; ++c)
+          if (c < kb) b = fma(-f[c], z[c], b);
       }
     }
     return tm.gather(b, act ? src : 0);

@@ -164,13 +164,13 @@ struct MAEval {
       const int j = e / NX, i = e - j * NX;
       A[e] = (i == j) ? 1.0 : 0.0;
     }
-    team_sync();
+    __syncthreads();  // global entries written by one lane, updated by another
     for (int e = tm.lane; e < M::JNZ; e += T) {
       const int r = TRANS ? M::J_J[e] : M::J_I[e];
       const int c = TRANS ? M::J_I[e] : M::J_J[e];
       A[(size_t)c * NX + r] += g * s.Jv[e];
     }
-    team_sync();
+    __syncthreads();
   }
   // u_r = sum_i S_ir v_i into s.u; v in s.v
   __device__ __forceinline__ void stoich_t() {

@@ -18,9 +18,22 @@
 
 namespace amx {
 
+// Workgroup barrier that orders LDS accesses only: the fences name the local
+// address space, so outstanding global loads and stores stay in flight across
+// it (a full __syncthreads() waits for them).  Used where the waves of a team
+// exchange data through LDS; global data written by one lane and read by
+// another needs __syncthreads().
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Orders a team's LDS traffic (the engine exchanges data between the lanes of
+// a team only through LDS).
 __device__ __forceinline__ void team_sync() {
 #if AMX_TEAM_LANES > 64
-  __syncthreads();
+  lds_barrier();
 #else
   // A team never spans more than one wavefront, so wavefront-scope fences
   // order its LDS traffic: they keep the compiler from moving LDS accesses
@@ -167,11 +180,11 @@ struct Team {
   __device__ __forceinline__ double combine(double wv, Op op) const {
     TeamXchg& x = team_xchg();
     if (__lane_id() == 0) x.d[lane >> 6] = wv;
-    __syncthreads();
+    lds_barrier();
     double r = x.d[0];
 #pragma unroll
     for (int w = 1; w < W; ++w) r = op(r, x.d[w]);
-    __syncthreads();
+    lds_barrier();
     return r;
   }
 
@@ -202,11 +215,11 @@ struct Team {
       const int wv = Wave64::maxi(v);
       TeamXchg& x = team_xchg();
       if (__lane_id() == 0) x.i[lane >> 6] = wv;
-      __syncthreads();
+      lds_barrier();
       int r = x.i[0];
 #pragma unroll
       for (int w = 1; w < W; ++w) r = x.i[w] > r ? x.i[w] : r;
-      __syncthreads();
+      lds_barrier();
       return r;
     } else if constexpr (T == 64) {
       return Wave64::maxi(v);
@@ -229,7 +242,7 @@ struct Team {
         x.d[lane >> 6] = wv;
         x.i[lane >> 6] = wi;
       }
-      __syncthreads();
+      lds_barrier();
       double bv = x.d[0];
       int bi = x.i[0];
 #pragma unroll
@@ -238,7 +251,7 @@ struct Team {
         const int ri = x.i[w];
         if (rv > bv || (rv == bv && ri < bi)) { bv = rv; bi = ri; }
       }
-      __syncthreads();
+      lds_barrier();
       idx = bi;
       return bv;
     } else if constexpr (T == 64) {
@@ -260,9 +273,9 @@ struct Team {
     if constexpr (T > 64) {
       TeamXchg& x = team_xchg();
       if (lane == src) x.d[0] = v;
-      __syncthreads();
+      lds_barrier();
       const double r = x.d[0];
-      __syncthreads();
+      lds_barrier();
       return r;
     } else if constexpr (T == 64) {
       return readlane_d(v, src);
@@ -274,9 +287,9 @@ struct Team {
     if constexpr (T > 64) {
       TeamXchg& x = team_xchg();
       if (lane == src) x.i[0] = v;
-      __syncthreads();
+      lds_barrier();
       const int r = x.i[0];
-      __syncthreads();
+      lds_barrier();
       return r;
     } else if constexpr (T == 64) {
       return __builtin_amdgcn_readlane(v, src);
@@ -289,9 +302,9 @@ struct Team {
     if constexpr (T > 64) {
       TeamXchg& x = team_xchg();
       x.d[lane] = v;
-      __syncthreads();
+      lds_barrier();
       const double r = x.d[src];
-      __syncthreads();
+      lds_barrier();
       return r;
     } else {
       return __shfl(v, src, T);

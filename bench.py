@@ -3,11 +3,14 @@
 atol 1e-12, FP64) on one MI355X, beside the plain GPU BDF (one instance per
 launch, forward sensitivities) and the CVODES CPU path.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config N40|N200]
 
-One step = one batched adjoint NLL+gradient evaluation of all 1024
-conditions (llh and the 120-entry sllh of each), inputs resident in HBM.
-Prints one JSON line.
+One step = one batched adjoint NLL+gradient evaluation of all conditions of
+the configuration (llh and the full sllh of each), inputs resident in HBM.
+--config N200 measures the 200-species / 600-reaction network over 256
+conditions (one instance per 256-lane workgroup) the same way; it has no
+plain-GPU-BDF line (that path needs forward sensitivities, which the
+workgroup engine does not provide).  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -54,10 +57,22 @@ def flop_model(counters: np.ndarray, n: int, ops: dict, nt: int, ny: int) -> tup
     return fwd, bwd
 
 
-def traffic_from_profile():
+CONFIGS = {
+    "N40": dict(counters=("n40_counters1024.npz", "counters_asa"), cpu_sample=128,
+                plain_sample=8, traffic="n40_adjoint_hbm_traffic.json",
+                workload="N40 adjoint NLL+gradient: 40 species, 120 reactions, 1024 input "
+                         "conditions, 25 times, 30 log-normal observables"),
+    "N200": dict(counters=("n200_conditions4.npz", "counters_bench"), cpu_sample=16,
+                 plain_sample=0, traffic="n200_adjoint_hbm_traffic.json",
+                 workload="N200 adjoint NLL+gradient: 200 species, 600 reactions, 256 input "
+                          "conditions, 40 times, 60 log-normal observables"),
+}
+
+
+def traffic_from_profile(name: str):
     """HBM bytes per launch of the dominant kernel from the committed
     rocprofv3 counter summary, if present."""
-    p = ROOT / "profiles" / "n40_adjoint_hbm_traffic.json"
+    p = ROOT / "profiles" / name
     if not p.exists():
         return None
     try:
@@ -66,7 +81,7 @@ def traffic_from_profile():
         return None
 
 
-def cpu_baseline(cs, theta, kfix, meas, sigma, sample: int):
+def cpu_baseline(cs, theta, kfix, meas, sigma, sample: int, config: str):
     """The reference's CVODES (oracle/_ref, built from the reference sources)
     on the host cores: adjoint NLL+gradient of the first `sample` conditions."""
     try:
@@ -82,7 +97,8 @@ def cpu_baseline(cs, theta, kfix, meas, sigma, sample: int):
     dt = time.perf_counter() - t0
     ok = int((r["status"] == 0).sum())
     return {"value": sample / dt, "unit": "evals/s", "cores": cores, "kind": "reference",
-            "sample": f"{sample} of 1024 N40 conditions, adjoint NLL+gradient, CVODES 7.1.1 "
+            "sample": f"{sample} of {theta.shape[0]} {config} conditions, adjoint NLL+gradient, "
+                      f"CVODES 7.1.1 "
                       f"(reference sources) + AMICI layer restated in C, OpenMP over conditions, "
                       f"{dt:.2f} s wall, {ok}/{sample} succeeded"}
 
@@ -92,10 +108,11 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--plain-sample", type=int, default=8,
-                    help="conditions timed on the plain GPU BDF path (0 = skip)")
-    ap.add_argument("--cpu-sample", type=int, default=128,
-                    help="conditions timed on the CPU baseline (0 = skip)")
+    ap.add_argument("--config", choices=sorted(CONFIGS), default="N40")
+    ap.add_argument("--plain-sample", type=int, default=None,
+                    help="conditions timed on the plain GPU BDF path (0 = skip; N40: 8)")
+    ap.add_argument("--cpu-sample", type=int, default=None,
+                    help="conditions timed on the CPU baseline (0 = skip; N40: 128, N200: 16)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES bytes 5 times (the known-size "
                          "kernel that scales the HBM counters) and exit")
@@ -124,7 +141,12 @@ def main() -> int:
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": n * 8, "launches": 5}))
         return 0
-    cs = networks.condition_set("N40")
+    cfg = CONFIGS[args.config]
+    if args.plain_sample is None:
+        args.plain_sample = cfg["plain_sample"]
+    if args.cpu_sample is None:
+        args.cpu_sample = cfg["cpu_sample"]
+    cs = networks.condition_set(args.config)
     net = cs["network"]
     model = sim.Model(net)
     nb, nt = cs["theta"].shape[0], len(cs["ts"])
@@ -133,7 +155,7 @@ def main() -> int:
     ts = torch.as_tensor(cs["ts"], device=dev)
 
     # synthetic data: this path's own observables at the nominal parameters,
-    # with multiplicative log-normal noise (sigma 0.1, seed 3)
+    # with the configuration's noise (log-normal, sigma 0.1, its seed)
     states = sim.Solver(**TOL)
     out = sim.simulate_batch(model, states, theta, kfix, ts, want_x=False)
     torch.cuda.synchronize()
@@ -177,16 +199,22 @@ def main() -> int:
     # that define the algorithmic work
     stats = sim.simulate_batch(model, adj, theta, kfix, ts, meas, sigma, want_x=False,
                                want_y=False)["stats"].cpu().numpy()
-    gold = np.load(ROOT / "tests" / "golden" / "n40_counters1024.npz")["counters_asa"]
+    # CVODES counters at rtol 1e-8: every N40 condition; for N200 the 4
+    # conditions of the fixture stand for the batch (per-instance mean x nb)
+    cf, ck = cfg["counters"]
+    gold = np.load(ROOT / "tests" / "golden" / cf)[ck]
     ops = codegen.op_counts(net)
     f_fwd, f_bwd = flop_model(gold, net.nx, ops, nt, net.ny)
+    if len(gold) != nb:
+        f_fwd = np.full(nb, f_fwd.mean())
+        f_bwd = np.full(nb, f_bwd.mean())
     fwd_ms = kt["fwd_ms"] / max(kt["fwd_launches"], 1)
     bwd_ms = kt["bwd_ms"] / max(kt["bwd_launches"], 1)
     dom = "bwd" if bwd_ms >= fwd_ms else "fwd"
     dom_flops = float((f_bwd if dom == "bwd" else f_fwd).sum())
     dom_ms = bwd_ms if dom == "bwd" else fwd_ms
     achieved = dom_flops / (dom_ms * 1e-3) / 1e12
-    tr = traffic_from_profile()
+    tr = traffic_from_profile(cfg["traffic"])
     roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
                 "traffic": (tr or {}).get("bytes_per_launch") if tr else None,
@@ -202,15 +230,14 @@ def main() -> int:
     if args.plain_sample > 0:
         plain, grad_rel = plain_gpu_bdf(model, theta, kfix, ts, meas, sigma, sllh, nb, args.plain_sample)
 
-    cpu = cpu_baseline(cs, cs["theta"], cs["kfix"], meas_np, sigma_np, args.cpu_sample) \
-        if args.cpu_sample > 0 else None
+    cpu = cpu_baseline(cs, cs["theta"], cs["kfix"], meas_np, sigma_np, args.cpu_sample,
+                       args.config) if args.cpu_sample > 0 else None
 
     line = {
         "metric": METRIC, "value": value, "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "N40 adjoint NLL+gradient: 40 species, 120 reactions, "
-                               "1024 input conditions, 25 times, 30 log-normal observables",
+        "config": {"workload": cfg["workload"],
                    "instances": nb, "np": net.np, "rtol": TOL["rtol"], "atol": TOL["atol"],
                    "sensitivity": "adjoint"},
         "instances_ok": n_ok,
