@@ -196,23 +196,26 @@ struct LURows {
 // column), and the rows not yet chosen take the rank-NB update
 // A22 -= L21 U12 with U12 staged in LDS.  Global loads are issued in
 // batches ahead of the arithmetic and the stores, so one memory latency
-// covers a batch.  The solves run block by block: the NB pivot values of a
-// block are published through LDS (double-buffered, one barrier per block),
-// every lane solves the small triangle, and the other rows update with
-// their NB factors of the block.
+// covers a batch.  The diagonal 16 x 16 blocks of the factors stay in LDS
+// for the solves, which run block by block: the NB pivot values of a block
+// are published through LDS (double-buffered, one barrier per block), every
+// lane solves the small triangle, and the other rows update with their NB
+// factors of the block.
 template <int NX, int T>
 struct LUBig {
   static_assert(NX > 64 && NX <= T, "LUBig: 64 < NX <= T");
   static constexpr int NB = 16;  // panel width / solve block
   static constexpr int CB = 8;   // trailing columns per batch of loads
   static constexpr int W = T / 64;
+  static constexpr int NBLK = (NX + NB - 1) / NB;
   struct Shared {
     double u[NB][T + CB];       // U12 rows of the panel (column k1 + j at index j)
-    double l11[2][NB][NB];      // factors of a panel's / block's pivot rows inside it
+    double diag[NBLK][NB][NB];  // factors of each panel's pivot rows inside the panel
+                                // (row c: pivot of step k0 + c; L left of c, U from c)
+    double dinv[NBLK][NB];      // inverse pivots
     double cand[2][W][NB + 1];  // per wavefront: best |value|, then that row's panel values
     int cidx[2][W];             // per wavefront: row of the best candidate
     double y[2][NB];            // solve: values of the block's pivot rows
-    double d[2][NB];            // solve: their inverse pivots
   };
   static __device__ __forceinline__ Shared& shared() {
     __shared__ Shared sh;
@@ -287,13 +290,14 @@ struct LUBig {
       for (int c = 0; c < NB; ++c)
         if (act && c < kb) A[(size_t)(k0 + c) * NX + i] = pa[c];
       const int k1 = k0 + kb;
-      if (k1 >= NX) break;
-      // factors of the panel's pivot rows inside the panel
+      // factors of the panel's pivot rows inside the panel, kept for the solves
       if (act && pos >= k0 && pos < k1) {
         const int c = pos - k0;
 #pragma unroll
-        for (int cc = 0; cc < NB; ++cc) sh.l11[0][c][cc] = pa[cc];
+        for (int cc = 0; cc < NB; ++cc) sh.diag[k0 / NB][c][cc] = pa[cc];
+        sh.dinv[k0 / NB][c] = dinv;
       }
+      if (k1 >= NX) break;
       __syncthreads();  // the panel's rows in global memory are read by other lanes next
       // U12: lane j finishes the pivot rows' entries of trailing column k1 + j
       const int col = k1 + i;
@@ -308,7 +312,7 @@ struct LUBig {
         for (int c = 0; c < NB; ++c)
           if (c < kb) {
 #pragma unroll
-            for (int cc = 0; cc < c; ++cc) u[c] = fma(-sh.l11[0][c][cc], u[cc], u[c]);
+            for (int cc = 0; cc < c; ++cc) u[c] = fma(-sh.diag[k0 / NB][c][cc], u[cc], u[c]);
           }
 #pragma unroll
         for (int c = 0; c < NB; ++c)
@@ -345,15 +349,15 @@ struct LUBig {
   }
 
   // Solves (LU) x = b with the factors in A; b is this lane's component.
-  // Every lane reads only its own row of A: the NB factors of a block are
-  // loaded together at the start of the block.
+  // The diagonal blocks of the factors come from LDS (kept by factor()), so
+  // a block's critical path is one LDS exchange of the pivot values; the
+  // other rows' loads of their own factors overlap it.
   template <class L>
   __device__ __forceinline__ double solve(double b, const Team<T>& tm, const double* A,
                                           const L& S) const {
     Shared& sh = shared();
     const int i = tm.lane;
     const bool act = i < NX;
-    constexpr int NBLK = (NX + NB - 1) / NB;
     int blk = 0;  // running block count: LDS buffer parity
     // forward substitution L y = P b, NB pivots at a time
     for (int kb0 = 0; kb0 < NBLK; ++kb0, ++blk) {
@@ -364,14 +368,8 @@ struct LUBig {
       const bool upd = act && pos >= k0 + kb;
       double f[NB];
 #pragma unroll
-      for (int c = 0; c < NB; ++c) f[c] = ((piv || upd) && c < kb) ? A[(size_t)(k0 + c) * NX + i] : 0.0;
-      if (piv) {
-        const int c = pos - k0;
-        sh.y[buf][c] = b;
-#pragma unroll
-        for (int cc = 0; cc < NB; ++cc)
-          if (cc < c) sh.l11[buf][c][cc] = f[cc];
-      }
+      for (int c = 0; c < NB; ++c) f[c] = (upd && c < kb) ? A[(size_t)(k0 + c) * NX + i] : 0.0;
+      if (piv) sh.y[buf][pos - k0] = b;
       lds_barrier();
       double y[NB];
 #pragma unroll
@@ -380,7 +378,7 @@ struct LUBig {
         if (c < kb) {
           v = sh.y[buf][c];
 #pragma unroll
-          for (int cc = 0; cc < c; ++cc) v = fma(-sh.l11[buf][c][cc], y[cc], v);
+          for (int cc = 0; cc < c; ++cc) v = fma(-sh.diag[kb0][c][cc], y[cc], v);
         }
         y[c] = v;
       }
@@ -401,15 +399,8 @@ struct LUBig {
       const bool upd = act && pos < k0;
       double f[NB];
 #pragma unroll
-      for (int c = 0; c < NB; ++c) f[c] = ((piv || upd) && c < kb) ? A[(size_t)(k0 + c) * NX + i] : 0.0;
-      if (piv) {
-        const int c = pos - k0;
-        sh.y[buf][c] = b;
-        sh.d[buf][c] = dinv;
-#pragma unroll
-        for (int cc = 0; cc < NB; ++cc)
-          if (cc > c && cc < kb) sh.l11[buf][c][cc] = f[cc];
-      }
+      for (int c = 0; c < NB; ++c) f[c] = (upd && c < kb) ? A[(size_t)(k0 + c) * NX + i] : 0.0;
+      if (piv) sh.y[buf][pos - k0] = b;
       lds_barrier();
       double z[NB];
 #pragma unroll
@@ -419,8 +410,8 @@ struct LUBig {
           v = sh.y[buf][c];
 #pragma unroll
           for (int cc = c + 1; cc < NB; ++cc)
-            if (cc < kb) v = fma(-sh.l11[buf][c][cc], z[cc], v);
-          v *= sh.d[buf][c];
+            if (cc < kb) v = fma(-sh.diag[kb0][c][cc], z[cc], v);
+          v *= sh.dinv[kb0][c];
         }
         z[c] = v;
       }
