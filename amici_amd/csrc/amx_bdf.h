@@ -216,6 +216,12 @@ struct LUBig {
     double cand[2][W][NB + 1];  // per wavefront: best |value|, then that row's panel values
     int cidx[2][W];             // per wavefront: row of the best candidate
     double y[2][NB];            // solve: values of the block's pivot rows
+    int perm[NX];               // perm[k] = row chosen at step k
+  };
+  // per-lane results of a factorisation
+  struct Fact {
+    int ok, pos, src;
+    double dinv;
   };
   static __device__ __forceinline__ Shared& shared() {
     __shared__ Shared sh;
@@ -226,16 +232,33 @@ struct LUBig {
   int pos;      // elimination step at which this lane's row was the pivot
   int src;      // lane that holds pivot row number `lane`
 
-  // Factorises A in place; false for a zero or non-finite pivot.
+  // The factorisation and the solve are separate (not inlined) functions
+  // that take and return everything by value: their loops then get the
+  // register file to themselves instead of competing with the integrator's
+  // live state, which is saved only around the call.
   template <class L>
-  __device__ __forceinline__ bool factor(const Team<T>& tm, double* A, L& S) {
+  __device__ __forceinline__ bool factor(const Team<T>&, double* A, L&) {
+    const Fact f = factor_impl(A);
+    pos = f.pos;
+    src = f.src;
+    dinv = f.dinv;
+    return f.ok != 0;
+  }
+  template <class L>
+  __device__ __forceinline__ double solve(double b, const Team<T>&, const double* A,
+                                          const L&) const {
+    return solve_impl(b, A, pos, src, dinv);
+  }
+
+  // Factorises A in place; ok = 0 for a zero or non-finite pivot.
+  static __device__ __noinline__ Fact factor_impl(double* A) {
     Shared& sh = shared();
-    const int i = tm.lane;
+    const int i = threadIdx.x;
     const int wave = i >> 6;
     const bool act = i < NX;
-    pos = act ? -1 : NX;
-    src = 0;
-    dinv = 0.0;
+    int pos = act ? -1 : NX;
+    int src = 0;
+    double dinv = 0.0;
     bool ok = true;
     for (int k0 = 0; k0 < NX && ok; k0 += NB) {
       const int kb = (NX - k0) < NB ? (NX - k0) : NB;
@@ -274,7 +297,7 @@ struct LUBig {
         double pr[NB];
 #pragma unroll
         for (int j = c; j < NB; ++j) pr[j] = sh.cand[buf][pw][1 + j];
-        if (i == 0) S.perm[k] = p;
+        if (i == 0) sh.perm[k] = p;
         if (i == k) src = p;
         const double inv = 1.0 / pr[c];
         if (i == p) { pos = k; dinv = inv; }
@@ -305,7 +328,7 @@ struct LUBig {
         int prr[NB];
         double u[NB];
 #pragma unroll
-        for (int c = 0; c < NB; ++c) prr[c] = (c < kb) ? S.perm[k0 + c] : 0;
+        for (int c = 0; c < NB; ++c) prr[c] = (c < kb) ? sh.perm[k0 + c] : 0;
 #pragma unroll
         for (int c = 0; c < NB; ++c) u[c] = (c < kb) ? A[(size_t)col * NX + prr[c]] : 0.0;
 #pragma unroll
@@ -345,17 +368,17 @@ struct LUBig {
     // U12 entries were written by other lanes than the rows' owners, which
     // read them in the solves
     __syncthreads();
-    return ok;
+    return Fact{ok ? 1 : 0, pos, src, dinv};
   }
 
   // Solves (LU) x = b with the factors in A; b is this lane's component.
   // The diagonal blocks of the factors come from LDS (kept by factor()), so
   // a block's critical path is one LDS exchange of the pivot values; the
   // other rows' loads of their own factors overlap it.
-  template <class L>
-  __device__ __forceinline__ double solve(double b, const Team<T>& tm, const double* A,
-                                          const L& S) const {
+  static __device__ __noinline__ double solve_impl(double b, const double* A, int pos, int src,
+                                                   double dinv) {
     Shared& sh = shared();
+    const Team<T> tm;
     const int i = tm.lane;
     const bool act = i < NX;
     int blk = 0;  // running block count: LDS buffer parity
