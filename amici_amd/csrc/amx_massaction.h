@@ -156,13 +156,14 @@ struct MAEval {
     }
     team_sync();
   }
-  // Newton matrix I + g * J (TRANS: I + g * J^T) of a big model into the
-  // column-major global matrix A (row i, column j at A[j * NX + i])
+  // d I + g J (TRANS: d I + g J^T) of a big model into the column-major
+  // global matrix A (row i, column j at A[j * NX + i]): the Newton matrix of
+  // the integrator (d = 1) or the Jacobian itself (d = 0, g = 1)
   template <bool TRANS>
-  __device__ __forceinline__ void newton_matrix(double g, double* A) {
+  __device__ __forceinline__ void newton_matrix(double g, double* A, double d = 1.0) {
     for (int e = tm.lane; e < NX * NX; e += T) {
       const int j = e / NX, i = e - j * NX;
-      A[e] = (i == j) ? 1.0 : 0.0;
+      A[e] = (i == j) ? d : 0.0;
     }
     __syncthreads();  // global entries written by one lane, updated by another
     for (int e = tm.lane; e < M::JNZ; e += T) {
@@ -684,6 +685,41 @@ __device__ __forceinline__ bool factor_jacobian(const Team<M::T>& tm, Smem<M>& s
   return lu.factor(tm, row, lds);
 }
 
+// LU of the Jacobian J(x) (or J^T) for the steady-state Newton steps and the
+// implicit-function sensitivities: row-per-lane in LDS for one-wavefront
+// teams, the blocked global-memory LU (matrix in `mat`) for workgroup teams.
+template <class M, bool BIG = big_model<M>()>
+struct JacLU {
+  LURows<M::NX, M::T> lu;
+  double* mat = nullptr;
+  template <bool TRANS, class L>
+  __device__ __forceinline__ bool factor(const Team<M::T>& tm, Smem<M>& s, MAEval<M>& ev, double x,
+                                         L& lds) {
+    return factor_jacobian<M, TRANS>(tm, s, ev, x, lu, lds);
+  }
+  template <class L>
+  __device__ __forceinline__ double solve(double b, const Team<M::T>& tm, L& lds) const {
+    return lu.solve(b, tm, lds);
+  }
+};
+template <class M>
+struct JacLU<M, true> {
+  LUBig<M::NX, M::T> lu;
+  double* mat = nullptr;
+  template <bool TRANS, class L>
+  __device__ __forceinline__ bool factor(const Team<M::T>& tm, Smem<M>&, MAEval<M>& ev, double x,
+                                         L& lds) {
+    ev.put_x(x);
+    ev.jacobian();
+    ev.template newton_matrix<TRANS>(1.0, mat, 0.0);
+    return lu.factor(tm, mat, lds);
+  }
+  template <class L>
+  __device__ __forceinline__ double solve(double b, const Team<M::T>& tm, L& lds) const {
+    return lu.solve(b, tm, mat, lds);
+  }
+};
+
 // Damped Newton on xdot(x) = 0 from x: the direction solves J delta = -xdot
 // (NewtonSolver::get_step), the damping factor doubles after a step that
 // lowers the residual and is quartered otherwise, failing below 1e-8
@@ -693,8 +729,8 @@ __device__ __forceinline__ bool factor_jacobian(const Team<M::T>& tm, Smem<M>& s
 // state; otherwise an AMICI status is returned.
 template <class M>
 __device__ __forceinline__ int steady_state(const BatchArgs& a, const Team<M::T>& tm, Smem<M>& s,
-                                            BdfLds<M::NX, 0>& lds, double& x) {
-  constexpr int NX = M::NX, T = M::T;
+                                            BdfLds<M::NX, 0>& lds, double& x, double* mat) {
+  constexpr int NX = M::NX;
   MAEval<M> ev(tm, s);
   const bool act = tm.lane < NX;
   const double x0 = x;
@@ -703,17 +739,18 @@ __device__ __forceinline__ int steady_state(const BatchArgs& a, const Team<M::T>
   if (best < 1.0) return AMX_SUCCESS;
   int status = AMX_NO_STEADY_STATE;
   {
-    LURows<NX, T> lu;
+    JacLU<M> jl;
+    jl.mat = mat;
     double xold = x, delta = 0.0, gam = 1.0;
     bool ok = true;
     for (int it = 0; it < a.newton_maxsteps; ++it) {
       if (ok) {
         xold = x;
-        if (!factor_jacobian<M, false>(tm, s, ev, x, lu, lds)) {
+        if (!jl.template factor<false>(tm, s, ev, x, lds)) {
           status = AMX_SINGULAR_JACOBIAN;
           break;
         }
-        delta = lu.solve(act ? -f : 0.0, tm, lds);
+        delta = jl.solve(act ? -f : 0.0, tm, lds);
         if (!act) delta = 0.0;
       }
       x = xold + gam * delta;
@@ -733,6 +770,7 @@ __device__ __forceinline__ int steady_state(const BatchArgs& a, const Team<M::T>
   }
   // integration from the initial state
   FwdProblem<M, 0> pp(tm, s, nullptr);
+  pp.mat = mat;
   BDF<FwdProblem<M, 0>> bdf(pp, lds, a.sp);
   if (a.maxtime_ticks) bdf.sp.deadline = wall_clock64() + a.maxtime_ticks;
   bdf.init(0.0, x0, nullptr);
@@ -772,13 +810,14 @@ __device__ __forceinline__ void simulate_forward(int inst, const BatchArgs& a, c
   int pre_status = AMX_SUCCESS;
   double xss = 0.0;
   constexpr bool BIG = big_model<M>();
-  if constexpr (!BIG) if (a.kfix_pre) {
+  double* const mat = BIG ? a.lu_ws + I * (size_t)NX * NX : nullptr;
+  if (a.kfix_pre) {
     double xp = 0.0;
     if (tm.lane < NX) {
       const int kx = M::X0K[tm.lane];
       xp = (kx >= 0) ? a.kfix_pre[I * NK + kx] : M::X0C[tm.lane];
     }
-    pre_status = steady_state<M>(a, tm, s, lds, xp);
+    pre_status = steady_state<M>(a, tm, s, lds, xp, mat);
     xss = xp;
     if (tm.lane < NX && M::X0K[tm.lane] < 0) x0 = xp;
     if constexpr (MODE == 2) {
@@ -796,7 +835,7 @@ __device__ __forceinline__ void simulate_forward(int inst, const BatchArgs& a, c
     pb0.tf = a.traj_f + I * (size_t)a.max_traj * NX;
     pb0.max_traj = a.max_traj;
   }
-  if constexpr (BIG) pb0.mat = a.lu_ws + I * (size_t)NX * NX;
+  pb0.mat = mat;
   PB& pb = pb0;
   BDF<PB> bdf(pb, lds, a.sp);
   if (a.maxtime_ticks) bdf.sp.deadline = wall_clock64() + a.maxtime_ticks;
@@ -967,14 +1006,15 @@ __device__ __forceinline__ void simulate_backward(int inst, const BatchArgs& a, 
   double gss[EPQ];
 #pragma unroll
   for (int e = 0; e < EPQ; ++e) gss[e] = 0.0;
-  if constexpr (!BIG) if (a.kfix_pre && status == AMX_SUCCESS) {
+  if (a.kfix_pre && status == AMX_SUCCESS) {
     const double xs = a.xss[I * NX + (tm.lane < NX ? tm.lane : 0)];
     const double xbm = (tm.lane < NX && M::X0K[tm.lane] < 0) ? xB : 0.0;
-    LURows<NX, T> lu;
-    if (!factor_jacobian<M, true>(tm, s, pb0.ev, xs, lu, lds)) {
+    JacLU<M> jl;
+    jl.mat = pb0.mat;
+    if (!jl.template factor<true>(tm, s, pb0.ev, xs, lds)) {
       status = AMX_SINGULAR_JACOBIAN;
     } else {
-      double v = lu.solve(xbm, tm, lds);
+      double v = jl.solve(xbm, tm, lds);
       if (tm.lane >= NX) v = 0.0;
       team_sync();
       if (tm.lane < NX) s.v[tm.lane] = v;

@@ -313,6 +313,12 @@ def network_n40_open() -> ReactionNetwork:
     return open_network(network_n40(), seed=5, name="n40open")
 
 
+def network_n200_open() -> ReactionNetwork:
+    """The 200-species network with degradation of every species and
+    synthesis of five (805 reactions, 805 rate constants; seed 6)."""
+    return open_network(network_n200(), seed=6, name="n200open")
+
+
 def linear_chain(n: int = 3, seed: int = 11) -> ReactionNetwork:
     """Closed linear chain X0 <-> X1 <-> ... (first-order reactions only).
 
@@ -401,6 +407,13 @@ def condition_set(name: str) -> dict:
         return dict(network=net, theta=np.repeat(net.theta_nominal[None, :], nb, 0),
                     kfix=input_conditions(net, nb, seed=5), kfix_preeq=np.ones((nb, net.nk)),
                     ts=np.linspace(0.0, 100.0, 25), sigma=0.1, noise_seed=5)
+    if name == "N200P":
+        # pre-equilibrated as N40P: steady state at input level 1, inputs reset
+        net = network_n200_open()
+        nb = 256
+        return dict(network=net, theta=np.repeat(net.theta_nominal[None, :], nb, 0),
+                    kfix=input_conditions(net, nb, seed=6), kfix_preeq=np.ones((nb, net.nk)),
+                    ts=np.linspace(0.0, 500.0, 40), sigma=0.1, noise_seed=6)
     if name == "N200":
         net = network_n200()
         nb = 256
@@ -408,6 +421,31 @@ def condition_set(name: str) -> dict:
                     kfix=input_conditions(net, nb, seed=4), ts=np.linspace(0.0, 500.0, 40),
                     sigma=0.1, noise_seed=4)
     raise KeyError(name)
+
+
+def mix_groups(total: int = 2048, seed: int = 5) -> list:
+    """The mixed workload: ``total`` conditions drawn 50 / 35 / 15 % from the
+    Robertson, 40-species and 200-species generators.  The two reaction
+    networks are the open variants, pre-equilibrated to steady state at input
+    level 1 (their closed forms have conserved moieties and no isolated steady
+    state); Robertson has no fixed parameters, so its conditions cannot be
+    pre-equilibrated and are simulated from the initial state.  Returns one
+    condition set per model type (as :func:`condition_set`)."""
+    n_r = total // 2
+    n_40 = int(round(0.35 * total))
+    n_200 = total - n_r - n_40
+    net_r = robertson()
+    n40, n200 = network_n40_open(), network_n200_open()
+    return [
+        dict(network=net_r, theta=robertson_parameter_sets(n_r, seed=seed), kfix=np.zeros((n_r, 0)),
+             ts=robertson_times(), sigma=0.01, noise_seed=seed),
+        dict(network=n40, theta=np.repeat(n40.theta_nominal[None, :], n_40, 0),
+             kfix=input_conditions(n40, n_40, seed=seed), kfix_preeq=np.ones((n_40, n40.nk)),
+             ts=np.linspace(0.0, 100.0, 25), sigma=0.1, noise_seed=seed),
+        dict(network=n200, theta=np.repeat(n200.theta_nominal[None, :], n_200, 0),
+             kfix=input_conditions(n200, n_200, seed=seed), kfix_preeq=np.ones((n_200, n200.nk)),
+             ts=np.linspace(0.0, 500.0, 40), sigma=0.1, noise_seed=seed),
+    ]
 
 
 def pick(values: Sequence, idx: Sequence[int]) -> list:

@@ -398,6 +398,32 @@ def simulate_batch(model: Model, solver: Solver, theta, kfix, ts, meas=None, sig
     return out
 
 
+def simulate_concurrent(groups: Sequence[dict]) -> list:
+    """Several batches, typically of different models, at the same time: each
+    group (the keyword arguments of :func:`simulate_batch`: model, solver,
+    theta, kfix, ts and optionally meas, sigma, kfix_preeq, want_*) runs on
+    its own HIP stream (default priority), so the kernels of different models
+    share the GPU; inputs are staged and outputs allocated on the group's
+    stream.  Waits for all groups and returns their result dicts in order.
+    Results equal those of running the groups one after another."""
+    if len({id(g["model"]) for g in groups}) != len(groups):
+        raise ValueError("one group per Model object (each model owns one workspace)")
+    dev = _device()
+    streams = [torch.cuda.Stream(device=dev) for _ in groups]
+    cur = torch.cuda.current_stream(dev)
+    outs = []
+    for g, st in zip(groups, streams):
+        st.wait_stream(cur)
+        with torch.cuda.stream(st):
+            kw = dict(g)
+            outs.append(simulate_batch(kw.pop("model"), kw.pop("solver"), kw.pop("theta"),
+                                       kw.pop("kfix"), kw.pop("ts"), stream=st, **kw))
+    for st in streams:
+        cur.wait_stream(st)
+    torch.cuda.synchronize(dev)
+    return outs
+
+
 def run_simulations(model: Model, solver: Solver, edata_list: Sequence[ExpData],
                     failfast: bool = True, num_threads: int = 1) -> list:
     """All conditions of ``edata_list`` as one GPU batch; returns one

@@ -10,7 +10,9 @@ the configuration (llh and the full sllh of each), inputs resident in HBM.
 --config N200 measures the 200-species / 600-reaction network over 256
 conditions (one instance per 256-lane workgroup) the same way; it has no
 plain-GPU-BDF line (that path needs forward sensitivities, which the
-workgroup engine does not provide).  Prints one JSON line.
+workgroup engine does not provide).  --config MIX runs the mixed workload
+(2048 conditions of three model types, pre-equilibrated where the network
+has a steady state) as three concurrent batches.  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -108,7 +110,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", choices=sorted(CONFIGS), default="N40")
+    ap.add_argument("--config", choices=sorted(CONFIGS) + ["MIX"], default="N40")
     ap.add_argument("--plain-sample", type=int, default=None,
                     help="conditions timed on the plain GPU BDF path (0 = skip; N40: 8)")
     ap.add_argument("--cpu-sample", type=int, default=None,
@@ -141,6 +143,8 @@ def main() -> int:
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": n * 8, "launches": 5}))
         return 0
+    if args.config == "MIX":
+        return run_mix(args, dev)
     cfg = CONFIGS[args.config]
     if args.plain_sample is None:
         args.plain_sample = cfg["plain_sample"]
@@ -252,6 +256,124 @@ def main() -> int:
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def run_mix(args, dev) -> int:
+    """The mixed workload (networks.mix_groups: 2048 conditions, 50 / 35 / 15 %
+    Robertson / open 40-species / open 200-species, the latter two
+    pre-equilibrated): one step runs the three model batches at the same
+    time, each on its own stream (sim.simulate_concurrent)."""
+    import torch
+
+    from amici_amd import networks, sim
+
+    groups, total = [], 0
+    for cs in networks.mix_groups():
+        net = cs["network"]
+        model = sim.Model(net)
+        theta = torch.as_tensor(cs["theta"], device=dev)
+        kfix = torch.as_tensor(cs["kfix"], device=dev)
+        ts = torch.as_tensor(cs["ts"], device=dev)
+        kpre = cs.get("kfix_preeq")
+        kpre = torch.as_tensor(kpre, device=dev) if kpre is not None else None
+        out = sim.simulate_batch(model, sim.Solver(**TOL), theta, kfix, ts, want_x=False,
+                                 kfix_preeq=kpre)
+        torch.cuda.synchronize()
+        if int((out["status"] != 0).sum()):
+            raise RuntimeError(f"data generation failed for {net.name}")
+        meas_np = networks.add_noise(out["y"].cpu().numpy(), net.noise, cs["sigma"],
+                                     seed=cs["noise_seed"])
+        adj = sim.Solver(**TOL)
+        adj.set_sensitivity_order(sim.SensitivityOrder.first)
+        adj.set_sensitivity_method(sim.SensitivityMethod.adjoint)
+        groups.append(dict(model=model, solver=adj, theta=theta, kfix=kfix, ts=ts,
+                           meas=torch.as_tensor(meas_np, device=dev),
+                           sigma=torch.full(meas_np.shape, cs["sigma"], dtype=torch.float64,
+                                            device=dev),
+                           kfix_preeq=kpre, want_x=False, want_y=False, want_stats=False,
+                           _cs=cs, _meas=meas_np))
+        total += theta.shape[0]
+    run = [{k: v for k, v in g.items() if not k.startswith("_")} for g in groups]
+
+    def step():
+        return sim.simulate_concurrent(run)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ok = sum(int((r["status"] == 0).sum()) for r in res)
+    # each group alone, one after another (same work, no overlap)
+    alone = []
+    for g in run:
+        kw = dict(g)
+        t1 = time.perf_counter()
+        sim.simulate_batch(kw.pop("model"), kw.pop("solver"), kw.pop("theta"), kw.pop("kfix"),
+                           kw.pop("ts"), **kw)
+        torch.cuda.synchronize()
+        alone.append(time.perf_counter() - t1)
+    cpu = None
+    if args.cpu_sample is None or args.cpu_sample > 0:
+        cpu = mix_cpu_baseline(groups)
+    line = {
+        "metric": METRIC, "value": total * args.steps / dt, "unit": "evals/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "MIX adjoint NLL+gradient: 2048 conditions, 1024 Robertson, "
+                               "717 open 40-species and 307 open 200-species (the last two "
+                               "pre-equilibrated), three model batches on concurrent streams",
+                   "instances": total, "rtol": TOL["rtol"], "atol": TOL["atol"],
+                   "sensitivity": "adjoint"},
+        "instances_ok": ok,
+        "groups": [{"model": g["model"].network.name, "instances": int(g["theta"].shape[0]),
+                    "alone_ms": a * 1e3} for g, a in zip(groups, alone)],
+        "concurrency_gain": sum(alone) / (dt / args.steps),
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def mix_cpu_baseline(groups):
+    """The reference's CVODES on the host cores over a sample of each group,
+    combined into the rate for the whole mix."""
+    try:
+        from oracle import cvodes_oracle as O
+        O.lib()
+    except Exception as e:  # library not built on this machine
+        log(f"cpu baseline skipped: {e}")
+        return None
+    cores = min(16, len(os.sched_getaffinity(0)))
+    samples = {"robertson": 128, "n40open": 32, "n200open": 16}
+    t_total, parts = 0.0, []
+    for g in groups:
+        cs = g["_cs"]
+        name = cs["network"].name
+        k = samples[name]
+        # the 200-species group is timed without its pre-equilibration (with it, CVODES
+        # integrates 805 forward sensitivities to t = 1e6: minutes per condition), so
+        # the CPU rate of the mix is an upper bound
+        kw = ({"kpre": cs["kfix_preeq"][:k]} if "kfix_preeq" in cs and name != "n200open"
+              else {})
+        t0 = time.perf_counter()
+        O.simulate(cs["network"], cs["theta"][:k], cs["kfix"][:k], cs["ts"], g["_meas"][:k],
+                   np.full(g["_meas"][:k].shape, cs["sigma"]), sensi=2, nthreads=cores, **TOL, **kw)
+        rate = k / (time.perf_counter() - t0)
+        t_total += cs["theta"].shape[0] / rate
+        parts.append(f"{name} {k} conditions {rate:.1f} evals/s")
+    total = sum(g["theta"].shape[0] for g in groups)
+    return {"value": total / t_total, "unit": "evals/s", "cores": cores, "kind": "reference",
+            "sample": "CVODES 7.1.1 (reference sources) + AMICI layer restated in C, OpenMP over "
+                      "conditions, adjoint with pre-equilibration for the open 40-species group "
+                      "(the open 200-species group without it: an upper bound on the CPU rate); "
+                      "per group: " + "; ".join(parts) + "; rate of the whole mix = "
+                      "2048 / sum(count / rate)"}
 
 
 def plain_gpu_bdf(model, theta, kfix, ts, meas, sigma, sllh, nb, S):

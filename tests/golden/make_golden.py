@@ -162,6 +162,25 @@ def n200() -> None:
                         counters_bench=c["counters"].astype(np.int32))
 
 
+def n200_preeq() -> None:
+    """4 conditions of the pre-equilibrated open 200-species network: steady
+    state at input level 1 (CVODES integrated to t = 1e6, with forward
+    sensitivities for the adjoint's steady-state term), inputs reset to the
+    condition's levels, then 40 outputs; states and adjoint gradients."""
+    cs = networks.condition_set("N200P")
+    net, ts = cs["network"], cs["ts"]
+    nb = 4
+    theta, kfix, kpre = cs["theta"][:nb], cs["kfix"][:nb], cs["kfix_preeq"][:nb]
+    y = O.simulate(net, theta, kfix, ts, nthreads=4, kpre=kpre, **TIGHT)["y"]
+    meas = networks.add_noise(y, net.noise, cs["sigma"], seed=cs["noise_seed"])
+    sigma = np.full_like(meas, cs["sigma"])
+    s = run(net, theta, kfix, ts, meas, sigma, 0, TIGHT, kpre=kpre)
+    a = run(net, theta, kfix, ts, meas, sigma, 2, TIGHT, kpre=kpre)
+    np.savez_compressed(OUT / "n200open_preeq4.npz", theta=theta, kfix=kfix, kfix_preeq=kpre,
+                        ts=ts, meas=meas, sigma=sigma, x=s["x"], y=s["y"], llh=s["llh"],
+                        sllh_asa=a["sllh"])
+
+
 def step_limit() -> None:
     """Robertson at the nominal parameters with a forward step budget
     (Solver::set_max_steps): the budget counts over the whole run
@@ -193,6 +212,8 @@ if __name__ == "__main__":
         step_limit()
     elif sys.argv[1:] == ["n200"]:
         n200()
+    elif sys.argv[1:] == ["n200_preeq"]:
+        n200_preeq()
     elif sys.argv[1:] == ["n40_preeq"]:
         n40_preeq()
     else:

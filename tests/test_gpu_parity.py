@@ -218,3 +218,38 @@ def test_n200_rejects_forward_sensitivities(n200_model):
     g = _load("n200_conditions4")
     with pytest.raises(RuntimeError, match="-22"):
         _run({"m": n200_model}, "m", g, sim.SensitivityMethod.forward)
+
+
+@pytest.mark.parametrize("meth", [0, 2])
+def test_n200_preequilibration(gpu, meth):
+    """Pre-equilibration of the open 200-species network on the workgroup
+    engine (Newton steps and the adjoint's steady-state term through the
+    global-memory LU of the Jacobian) against the oracle's CVODES integration
+    to t = 1e6, for states and adjoint gradients."""
+    import torch
+    from amici_amd import sim
+    model = sim.Model(networks.network_n200_open())
+    g = _load("n200open_preeq4")
+    sol = sim.Solver(**TIGHT)
+    # AMICI's default acceptance (WRMS of dx/dt < 1 at 100 x the tolerances)
+    # leaves a state error of about |dx/dt| / lambda_min, 1e-6 relative for
+    # this network's slowest degradation rates (1e-2): the oracle's steady
+    # state is integrated to t = 1e6 at rtol 1e-10, so the comparison asks
+    # for a converged steady state
+    sol.set_relative_tolerance_steady_state(1e-12)
+    sol.set_absolute_tolerance_steady_state(1e-16)
+    if meth:
+        sol.set_sensitivity_order(sim.SensitivityOrder.first)
+        sol.set_sensitivity_method(meth)
+    out = sim.simulate_batch(model, sol, g["theta"], g["kfix"], g["ts"], g["meas"], g["sigma"],
+                             kfix_preeq=g["kfix_preeq"])
+    torch.cuda.synchronize()
+    out = {k: (v.cpu().numpy() if hasattr(v, "cpu") else v) for k, v in out.items()
+           if k != "_keepalive"}
+    _dump(f"preeq_n200_{meth}", out)
+    assert (out["status"] == 0).all(), out["status"]
+    _close(out["x"], g["x"], Y_RTOL, Y_FLOOR, "x")
+    _close(out["y"], g["y"], Y_RTOL, Y_FLOOR, "y")
+    _check_llh(model.network, out, g)
+    if meth:
+        _close(out["sllh"], g["sllh_asa"], 1e-5, 1e-7, "sllh (adjoint)")

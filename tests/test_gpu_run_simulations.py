@@ -241,3 +241,27 @@ def test_failfast_and_mixed_grids(sim_models):
     alone = sim.run_simulation(model, _solver(), e_short)
     np.testing.assert_array_equal(mixed[1].x, alone.x)
     np.testing.assert_array_equal(mixed[0].x, r_all[0].x)
+
+
+def test_concurrent_groups_equal_sequential(sim_models):
+    """Two models' batches on concurrent streams give the same results, bit
+    for bit, as the batches run one after another."""
+    import torch
+    from amici_amd import sim
+    gr, gn = _load("robertson_batch64"), _load("n40_conditions16")
+    sol = _solver(sim.SensitivityMethod.adjoint, rtol=1e-8, atol=1e-12, quad_rtol=1e-8,
+                  quad_atol=1e-12)
+    groups = [dict(model=sim_models["robertson"], solver=sol, theta=gr["theta"], kfix=gr["kfix"],
+                   ts=gr["ts"], meas=gr["meas"], sigma=gr["sigma"]),
+              dict(model=sim_models["n40"], solver=sol, theta=gn["theta"], kfix=gn["kfix"],
+                   ts=gn["ts"], meas=gn["meas"], sigma=gn["sigma"])]
+    conc = sim.simulate_concurrent(groups)
+    for g, c in zip(groups, conc):
+        kw = dict(g)
+        alone = sim.simulate_batch(kw.pop("model"), kw.pop("solver"), kw.pop("theta"), kw.pop("kfix"),
+                                   kw.pop("ts"), **kw)
+        torch.cuda.synchronize()
+        for k in ("llh", "sllh", "status", "x"):
+            np.testing.assert_array_equal(c[k].cpu().numpy(), alone[k].cpu().numpy(), err_msg=k)
+    with pytest.raises(ValueError):
+        sim.simulate_concurrent([groups[0], groups[0]])
