@@ -10,7 +10,8 @@ the configuration (llh and the full sllh of each), inputs resident in HBM.
 --config N200 measures the 200-species / 600-reaction network over 256
 conditions (one instance per 256-lane workgroup) the same way; it has no
 plain-GPU-BDF line (that path needs forward sensitivities, which the
-workgroup engine does not provide).  --config MIX runs the mixed workload
+workgroup engine does not provide).  --config RB measures 4096 Robertson
+parameter sets with forward sensitivities.  --config MIX runs the mixed workload
 (2048 conditions of three model types, pre-equilibrated where the network
 has a steady state) as three concurrent batches.  Prints one JSON line.
 """
@@ -110,7 +111,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", choices=sorted(CONFIGS) + ["MIX"], default="N40")
+    ap.add_argument("--config", choices=sorted(CONFIGS) + ["MIX", "RB"], default="N40")
     ap.add_argument("--plain-sample", type=int, default=None,
                     help="conditions timed on the plain GPU BDF path (0 = skip; N40: 8)")
     ap.add_argument("--cpu-sample", type=int, default=None,
@@ -145,6 +146,8 @@ def main() -> int:
         return 0
     if args.config == "MIX":
         return run_mix(args, dev)
+    if args.config == "RB":
+        return run_rb(args, dev)
     cfg = CONFIGS[args.config]
     if args.plain_sample is None:
         args.plain_sample = cfg["plain_sample"]
@@ -333,6 +336,99 @@ def run_mix(args, dev) -> int:
         "groups": [{"model": g["model"].network.name, "instances": int(g["theta"].shape[0]),
                     "alone_ms": a * 1e3} for g, a in zip(groups, alone)],
         "concurrency_gain": sum(alone) / (dt / args.steps),
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def run_rb(args, dev) -> int:
+    """RB: 4096 Robertson parameter sets (log10 k ~ N(nominal, 0.5)), forward
+    sensitivities of llh with respect to the 3 rate constants, one batch per
+    step; beside it the plain GPU BDF (one launch per set) and CVODES on the
+    host cores, both on a sample."""
+    import torch
+
+    from amici_amd import networks, sim
+
+    cs = networks.condition_set("RB")
+    net = cs["network"]
+    model = sim.Model(net)
+    nb = cs["theta"].shape[0]
+    theta = torch.as_tensor(cs["theta"], device=dev)
+    kfix = torch.as_tensor(cs["kfix"], device=dev)
+    ts = torch.as_tensor(cs["ts"], device=dev)
+    # data: the nominal parameters' observables with N(0, 0.01^2) noise (seed 2)
+    nom = torch.as_tensor(np.repeat(net.theta_nominal[None, :], nb, 0), device=dev)
+    out = sim.simulate_batch(model, sim.Solver(**TOL), nom, kfix, ts, want_x=False)
+    torch.cuda.synchronize()
+    meas_np = networks.add_noise(out["y"].cpu().numpy(), net.noise, cs["sigma"], seed=cs["noise_seed"])
+    sigma_np = np.full_like(meas_np, cs["sigma"])
+    meas, sigma = torch.as_tensor(meas_np, device=dev), torch.as_tensor(sigma_np, device=dev)
+    fsa = sim.Solver(**TOL)
+    fsa.set_sensitivity_order(sim.SensitivityOrder.first)
+    fsa.set_sensitivity_method(sim.SensitivityMethod.forward)
+
+    def step():
+        return sim.simulate_batch(model, fsa, theta, kfix, ts, meas, sigma, want_x=False,
+                                  want_y=False, want_stats=False)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    model.lib.set_timing(True)
+    model.lib.kernel_times()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    kt = model.lib.kernel_times()
+    model.lib.set_timing(False)
+    status = res["status"].cpu().numpy()
+    ok = int((status == 0).sum())
+    S = args.plain_sample if args.plain_sample else 64
+    sl = slice(0, S)
+    sim.simulate_batch(model, fsa, theta[:1], kfix[:1], ts, meas[:1], sigma[:1], sequential=True)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    pl = sim.simulate_batch(model, fsa, theta[sl], kfix[sl], ts, meas[sl], sigma[sl], sequential=True,
+                            want_x=False, want_y=False)
+    torch.cuda.synchronize()
+    plain_dt = time.perf_counter() - t1
+    plain = {"value": S / plain_dt, "unit": "evals/s",
+             "sample": f"first {S} of {nb} parameter sets, "
+                       f"{int((pl['status'] == 0).sum())}/{S} succeeded, one launch each"}
+    cpu = None
+    ns = args.cpu_sample if args.cpu_sample else 1024
+    try:
+        from oracle import cvodes_oracle as O
+        O.lib()
+        cores = min(16, len(os.sched_getaffinity(0)))
+        t2 = time.perf_counter()
+        r = O.simulate(net, cs["theta"][:ns], cs["kfix"][:ns], cs["ts"], meas_np[:ns], sigma_np[:ns],
+                       sensi=1, nthreads=cores, **TOL)
+        cdt = time.perf_counter() - t2
+        cpu = {"value": ns / cdt, "unit": "evals/s", "cores": cores, "kind": "reference",
+               "sample": f"{ns} of {nb} RB parameter sets, forward sensitivities, CVODES 7.1.1 "
+                         f"(reference sources) + AMICI layer restated in C, {cdt:.2f} s wall, "
+                         f"{int((r['status'] == 0).sum())}/{ns} succeeded"}
+    except Exception as e:  # library not built on this machine
+        log(f"cpu baseline skipped: {e}")
+    value = nb * args.steps / dt
+    line = {
+        "metric": METRIC, "value": value, "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "RB forward-sensitivity NLL+gradient: Robertson, 4096 parameter sets "
+                               "(log10 k ~ N(nominal, 0.5)), 40 times, 2 observables",
+                   "instances": nb, "np": net.np, "rtol": TOL["rtol"], "atol": TOL["atol"],
+                   "sensitivity": "forward"},
+        "instances_ok": ok,
+        "status_counts": {str(int(k)): int(v) for k, v in zip(*np.unique(status, return_counts=True))},
+        "fwd_kernel_ms": kt["fwd_ms"] / max(kt["fwd_launches"], 1),
+        "plain_gpu_bdf": plain, "speedup_vs_plain_gpu_bdf": value / plain["value"],
         "roofline": None,
         "cpu_baseline": cpu,
     }
