@@ -21,11 +21,12 @@ def _declared():
     return sorted(set(re.findall(r"^\s*(?:int|size_t|const char\*)\s+(amx_\w+)\s*\(", text, re.M)))
 
 
-@pytest.fixture(scope="module", params=["robertson", "n40", "linear3", "n40open", "n200"])
+@pytest.fixture(scope="module", params=["robertson", "n40", "linear3", "n40open", "n200", "n200open"])
 def lib(request):
     net = {"robertson": networks.robertson, "n40": networks.network_n40,
            "linear3": lambda: networks.linear_chain(3),
-           "n40open": networks.network_n40_open, "n200": networks.network_n200}[request.param]()
+           "n40open": networks.network_n40_open, "n200": networks.network_n200,
+           "n200open": networks.network_n200_open}[request.param]()
     so = codegen.compile_model(net)
     return net, _lib.ModelLib(so)
 
