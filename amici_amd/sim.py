@@ -151,6 +151,27 @@ class Model:
     def create_solver(self) -> "Solver":
         return Solver()
 
+    def simulate(self, *, solver: "Solver | None" = None, edata=None, failfast: bool = True,
+                 num_threads: int = 1, sensi_method=None, sensi_order=None):
+        """``Model.simulate`` (python/sdist/amici/sim/sundials/_swig_wrappers.py:296-331):
+        one ExpData (or none) -> one ReturnData, a sequence -> a list.  The
+        sensitivity method / order ("forward", "adjoint", "first", or the
+        enums) apply to this call only; the solver's own settings are restored."""
+        solver = solver if solver is not None else self.create_solver()
+        old = (solver.sensi_meth, solver.sensi)
+        try:
+            if sensi_method is not None:
+                solver.set_sensitivity_method(SensitivityMethod[sensi_method]
+                                              if isinstance(sensi_method, str) else sensi_method)
+            if sensi_order is not None:
+                solver.set_sensitivity_order(SensitivityOrder[sensi_order]
+                                             if isinstance(sensi_order, str) else sensi_order)
+            if isinstance(edata, Sequence):
+                return run_simulations(self, solver, edata, failfast=failfast, num_threads=num_threads)
+            return run_simulation(self, solver, edata)
+        finally:
+            solver.sensi_meth, solver.sensi = old
+
     def workspace(self, nbytes: int, device) -> torch.Tensor:
         ws = self._ws.get(device)
         if ws is None or ws.numel() < nbytes:

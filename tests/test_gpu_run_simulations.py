@@ -265,3 +265,22 @@ def test_concurrent_groups_equal_sequential(sim_models):
             np.testing.assert_array_equal(c[k].cpu().numpy(), alone[k].cpu().numpy(), err_msg=k)
     with pytest.raises(ValueError):
         sim.simulate_concurrent([groups[0], groups[0]])
+
+
+def test_model_simulate(sim_models):
+    """Model.simulate (python/sdist/amici/sim/sundials/_swig_wrappers.py:296-331):
+    a list of ExpData runs run_simulations, one ExpData run_simulation; the
+    sensitivity settings apply to the call only."""
+    from amici_amd import sim
+    model = sim_models["robertson"]
+    g = _load("robertson_batch64")
+    ed = _edatas(model, g, range(4))
+    solver = _solver()
+    many = model.simulate(solver=solver, edata=ed, sensi_method="adjoint", sensi_order="first")
+    ref = sim.run_simulations(model, _solver(sim.SensitivityMethod.adjoint), ed)
+    for a, b in zip(many, ref):
+        np.testing.assert_array_equal(a.sllh, b.sllh)
+    assert solver.get_sensitivity_order() == sim.SensitivityOrder.none
+    one = model.simulate(solver=solver, edata=ed[1])
+    assert one.sllh is None and one.status == 0
+    np.testing.assert_array_equal(one.x, ref[1].x)
