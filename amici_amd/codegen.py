@@ -157,9 +157,9 @@ def emit_model_traits(net: ReactionNetwork, team: int | None = None) -> str:
     """The AmxModel traits struct (dimensions, team size, sparsity tables)."""
     t = model_tables(net)
     team = team or default_team(net.nx)
-    if team < net.nx or not (team in (128, 256) or (team <= 64 and team & (team - 1) == 0)):
-        raise ValueError(f"team size {team} does not fit nx={net.nx} (1 lane per state; "
-                         "a power of two up to 64, or 128 / 256 for a workgroup team)")
+    if team < net.nx or team not in (64, 128, 256):
+        raise ValueError(f"team size {team} does not fit nx={net.nx} (one lane per state; "
+                         "a team is one 64-lane wavefront or a workgroup of 128 / 256 lanes)")
     lines = [
         f"// Generated model module for network '{net.name}' (amici_amd.codegen).\n",
         "// Sparsity tables of xdot = S w, J, J^T v, df/dp and y = C x; see\n",

@@ -499,7 +499,9 @@ struct BDF {
       gammap, gamrat, rl1, crate, crateS, delp, acnrm, acnrmS, saved_tq5, hu,
       tolsf, tstop, dsm_last, dsm_test;
   int q, qprime, L_, qwait, nst, nstlp, nstlj, nscon, convfail;
-  bool jcur, tstopset;
+  bool jcur, tstopset, at_tstop;
+  // values returned by advance(): this lane's state and quadratures
+  double yout, yQout[EPQ];
   // counters
   int nfe, nsetups, nje, nni, netf, ncfn, nfQe, nniS;
   // optional step trace (team lane 0 writes records of TRACE_REC doubles)
@@ -601,12 +603,14 @@ struct BDF {
     nst = 0; nstlp = 0; nstlj = 0; nscon = 0; saved_tq5 = 0.0;
     gammap = 0.0; gamma = 0.0; gamrat = 1.0; crate = 1.0; crateS = 1.0;
     delp = 0.0; acnrm = 0.0; acnrmS = 0.0; convfail = NO_FAILURES;
-    jcur = false; tstopset = false; tstop = 0.0; etaq = etaqm1 = etaqp1 = 0.0;
+    jcur = false; tstopset = false; at_tstop = false; tstop = 0.0;
+    etaq = etaqm1 = etaqp1 = 0.0;
+    yout = active ? y0 : 0.0;
     nfe = nsetups = nje = nni = netf = ncfn = nfQe = nniS = 0;
     dsm_last = 0.0; dsm_test = 0.0;
     ewt = 0.0; acor = 0.0; y = active ? y0 : 0.0; ftemp = 0.0;
 #pragma unroll
-    for (int e = 0; e < EPQ; ++e) ewtQ[e] = acorQ[e] = yQ[e] = 0.0;
+    for (int e = 0; e < EPQ; ++e) ewtQ[e] = acorQ[e] = yQ[e] = yQout[e] = 0.0;
   }
 
   // ---------------- Nordsieck array manipulation ----------------
@@ -1308,13 +1312,52 @@ struct BDF {
     return AMX_SUCCESS;
   }
 
-  // ---------------- CVode(NORMAL) with tstop == tout ----------------
-  // Advances to tout; on success tn == tout and zn[0] holds y(tout).
-  // pb.on_step() is invoked for the initial point and after every accepted
-  // step (trajectory storage); returning false stops with AMX_TRAJ_FULL.
-  __device__ __forceinline__ int advance(double tout) {
-    tstop = tout;
+  // ---------------- output: zn[0] or the Nordsieck interpolant ----------------
+  __device__ __forceinline__ void out_zn0() {
+    yout = Z(0);
+#pragma unroll
+    for (int e = 0; e < EPQ; ++e) yQout[e] = ZQ(0, e);
+  }
+  // y(t) = sum_j zn[j] s^j with s = (t - tn) / h (CVodeGetDky / CVodeGetQuadDky
+  // with k = 0, cvodes.c:3742-3820, 3864-...): terms from j = q down to 0.
+  __device__ __forceinline__ void out_dky(double t) {
+    const double s = (t - tn) / h;
+    double c[LMAX];
+    c[0] = 1.0;
+#pragma unroll
+    for (int j = 1; j < LMAX; ++j) c[j] = c[j - 1] * s;
+    double acc = 0.0;
+    double accQ[EPQ];
+#pragma unroll
+    for (int e = 0; e < EPQ; ++e) accQ[e] = 0.0;
+#pragma unroll
+    for (int j = QMAX; j >= 0; --j) {
+      if (j <= q) {
+        acc = (j == q) ? c[j] * Z(j) : fma(c[j], Z(j), acc);
+        if constexpr (QUAD)
+#pragma unroll
+          for (int e = 0; e < EPQ; ++e)
+            accQ[e] = (j == q) ? c[j] * ZQ(j, e) : fma(c[j], ZQ(j, e), accQ[e]);
+      }
+    }
+    yout = acc;
+#pragma unroll
+    for (int e = 0; e < EPQ; ++e) yQout[e] = accQ[e];
+  }
+
+  // ---------------- CVode(tout, itask) with tstop set ----------------
+  // cvodes.c:2890-3620.  NORMAL mode (one_step false) returns at tstop with
+  // zn[0], or, once a step reaches or passes tout, with the interpolant at
+  // tout; ONE_STEP mode returns after one step.  The returned values are in
+  // yout / yQout (at_tstop tells which case).  mxstep bounds the steps of
+  // this call (CVodeSetMaxNumSteps).  pb.on_step() is invoked for the
+  // initial point and after every accepted step (trajectory storage);
+  // returning false stops with AMX_TRAJ_FULL.
+  __device__ __forceinline__ int advance(double tout, double tstop_, int mxstep,
+                                         bool one_step = false) {
+    tstop = tstop_;
     tstopset = true;
+    at_tstop = false;
     if (nst == 0) {
       set_ewt();
       if constexpr (QUAD) set_ewtQ();
@@ -1332,7 +1375,8 @@ struct BDF {
       if constexpr (SENS) pb.sens_init_deriv(tn, Z(0), f0);
       if (any_nonfinite(f0)) return AMX_RHSFUNC_FAIL;
       if ((tstop - tn) * (tout - tn) <= 0.0) return AMX_ILL_INPUT;
-      const int hf = hin(tout);
+      const double tout_hin = ((tout - tn) * (tout - tstop) > 0.0) ? tstop : tout;
+      const int hf = hin(tout_hin);
       if (hf != AMX_SUCCESS) return hf;
       if ((tn + h - tstop) * h > 0.0) h = (tstop - tn) * (1.0 - 4.0 * UROUND);
       hscale = h;
@@ -1346,21 +1390,28 @@ struct BDF {
     } else {
       const double troundoff = FUZZ_FACTOR * UROUND * (fabs(tn) + fabs(h));
       if (fabs(tn - tstop) <= troundoff) {
-        tstopset = false;
-        return AMX_SUCCESS;
-      }
-      if ((tn + hprime - tstop) * h > 0.0) {
+        if ((tout - tstop) * h >= 0.0 || fabs(tout - tstop) <= troundoff) {
+          tstopset = false;
+          at_tstop = true;
+          out_zn0();
+          return AMX_SUCCESS;
+        }
+      } else if ((tn + hprime - tstop) * h > 0.0) {
         hprime = (tstop - tn) * (1.0 - 4.0 * UROUND);
         eta = hprime / h;
       }
+      if (!one_step && (tn - tout) * h >= 0.0) {
+        out_dky(tout);
+        return AMX_SUCCESS;
+      }
     }
-    for (;;) {
+    for (int nstloc = 0;; ++nstloc) {
       if (nst > 0) {
         set_ewt();
         if constexpr (QUAD) if (quad_errcon()) set_ewtQ();
         if constexpr (SENS) pb.sens_set_ewt();
       }
-      if (nst >= sp.maxsteps) return AMX_TOO_MUCH_WORK;
+      if (nstloc >= mxstep) return AMX_TOO_MUCH_WORK;
       if (sp.deadline != 0 && wall_clock64() > sp.deadline) return AMX_MAX_TIME_EXCEEDED;
       double nrm = wrms(Z(0), ewt);
       if constexpr (QUAD) {
@@ -1380,15 +1431,61 @@ struct BDF {
       const double troundoff = FUZZ_FACTOR * UROUND * (fabs(tn) + fabs(h));
       if (fabs(tn - tstop) <= troundoff) tn = tstop;
       if (!pb.on_step(tn, Z(0), Z(1), hscale)) return AMX_TRAJ_FULL;
-      if (tn == tstop) {
-        tstopset = false;
-        return AMX_SUCCESS;
-      }
-      if ((tn + hprime - tstop) * h > 0.0) {
+      if (fabs(tn - tstop) <= troundoff) {
+        if ((tout - tstop) * h >= 0.0 || fabs(tout - tstop) <= troundoff) {
+          tstopset = false;
+          at_tstop = true;
+          out_zn0();
+          return AMX_SUCCESS;
+        }
+      } else if ((tn + hprime - tstop) * h > 0.0) {
         hprime = (tstop - tn) * (1.0 - 4.0 * UROUND);
         eta = hprime / h;
       }
+      if (!one_step && (tn - tout) * h >= 0.0) {
+        out_dky(tout);
+        return AMX_SUCCESS;
+      }
+      if (one_step) {
+        out_zn0();
+        return AMX_SUCCESS;
+      }
     }
+  }
+
+  // ---------------- restart at a data point (AMICI's backward problem) ----------------
+  // CVodeSolver::reset_state (src/solver_cvodes.cpp:471-505): order 1 and a
+  // new zn[0] (and znQ[0], reinit_quad_b :635-644) at time t; every other
+  // integrator variable keeps its value (step-size history, etamax, counters,
+  // the linear-solver bookkeeping).
+  __device__ __forceinline__ void reset_state(double t, double y0, const double (&q0)[EPQ]) {
+    tn = t;
+    q = 1;
+    L_ = 2;
+    qwait = L_;
+    hu = 0.0;
+    tolsf = 1.0;
+    team_sync();
+    setZ(0, y0);
+    if constexpr (QUAD)
+#pragma unroll
+      for (int e = 0; e < EPQ; ++e) setZQ(0, e, q0[e]);
+    team_sync();
+  }
+  // CVodeSolver::reInit_post_process (src/solver_cvodes.cpp:527-560): one
+  // ONE_STEP call towards tnext with nst set to 0, so CVode's first-call
+  // initialisation runs (f0, initial step from cvHin, and, since nst is 0 in
+  // that step, a fresh Jacobian and Newton matrix with nstlp = nstlj = 0);
+  // afterwards nst continues from its old value + 1.  A step that already
+  // lands on tnext is an error (CV_TSTOP_RETURN is not CV_SUCCESS there).
+  __device__ __forceinline__ int reinit_step(double tnext, int mxstep) {
+    const int nst_tmp = nst;
+    nst = 0;
+    const int r = advance(tnext, tnext, mxstep, true);
+    if (r != AMX_SUCCESS) return r;
+    if (at_tstop) return AMX_ERROR;
+    nst = nst_tmp + 1;
+    return AMX_SUCCESS;
   }
 };
 

@@ -36,6 +36,7 @@ struct BatchArgs {
   double* sllh;            // [nb][NP] (FSA / adjoint)
   int* status;             // [nb]
   int* stats;              // [nb][AMX_NSTATS] or null
+  int* diag;               // [nb][nt][AMX_NDIAG] per-output-time counters or null
   // adjoint forward trajectory (Hermite data at every accepted step)
   double* traj_t;          // [nb][max_traj]
   double* traj_y;          // [nb][max_traj][NX]
@@ -60,6 +61,10 @@ struct BatchArgs {
 // stats layout: forward nst, nfe, nsetups, nje, nni, netf, ncfn, last order,
 //               backward nstB, nfeB, nsetupsB, njeB, nniB, netfB, ncfnB, nfQeB
 constexpr int AMX_NSTATS = 16;
+// per-output-time diagnostics: forward numsteps, rhs evals, error-test
+// failures, nonlinear-solver convergence failures, last order; backward
+// numsteps, rhs evals, error-test failures, convergence failures
+constexpr int AMX_NDIAG = 9;
 constexpr int SENS_SLOTS = LMAX + 3;  // znS[0..5], acorS, ewtS, sllh partials
 
 // Models with more than 64 states run one instance per workgroup (a
@@ -91,7 +96,8 @@ struct Launch {
   static constexpr int TEAMS = big_model<M>() ? 1
                                : (BY_THREADS < BY_LDS ? BY_THREADS : BY_LDS) > 0
                                    ? (BY_THREADS < BY_LDS ? BY_THREADS : BY_LDS) : 1;
-  static constexpr int THREADS = TEAMS * M::T < 64 ? 64 : TEAMS * M::T;
+  static constexpr int THREADS = TEAMS * M::T;
+  static_assert(M::T % 64 == 0 && THREADS <= 256, "whole wavefronts per team, <= 256 threads");
 };
 
 // ---------------------------------------------------------------------------
@@ -776,9 +782,9 @@ __device__ __forceinline__ int steady_state(const BatchArgs& a, const Team<M::T>
   bdf.init(0.0, x0, nullptr);
   double tout = 1.0;
   for (int k = 0; k < 16; ++k, tout *= 10.0) {
-    const int st = bdf.advance(tout);
+    const int st = bdf.advance(tout, tout, a.sp.maxsteps - bdf.nst);
     if (st != AMX_SUCCESS) return status == AMX_NO_STEADY_STATE ? st : status;
-    const double xv = bdf.Z(0);
+    const double xv = bdf.yout;
     const double fv = ev.xdot(xv);
     if (ss_wrms<M>(tm, fv, xv, a) < 1.0) {
       x = xv;
@@ -877,10 +883,22 @@ __device__ __forceinline__ void simulate_forward(int inst, const BatchArgs& a, c
         status = AMX_ERROR;
         break;
       }
-      status = bdf.advance(tout);
+      status = bdf.advance(tout, tout, bdf.sp.maxsteps - bdf.nst);
       if (status != AMX_SUCCESS) break;
     }
-    const double x = bdf.Z(0);
+    const double x = bdf.yout;
+    if (a.diag && tm.lane == 0) {
+      // Solver::store_diagnosis (src/solver.cpp:349-375): cumulative counters,
+      // zeros before the solver was first called
+      int* d = a.diag + (I * a.nt + it) * AMX_NDIAG;
+      const bool called = bdf.nst > 0;
+      d[0] = called ? bdf.nst : 0;
+      d[1] = called ? bdf.nfe : 0;
+      d[2] = called ? bdf.netf : 0;
+      d[3] = called ? bdf.ncfn : 0;
+      d[4] = called ? bdf.q : 0;
+      for (int k = 5; k < AMX_NDIAG; ++k) d[k] = 0;  // backward slots (adjoint sweep)
+    }
     if (a.x && tm.lane < NX) a.x[(I * a.nt + it) * NX + tm.lane] = x;
     const size_t mo = (I * a.nt + it) * NY;
     const ObjOut ob = objective_at<M>(tm, s, x, a.meas ? a.meas + mo : nullptr, a.sigma + mo,
@@ -901,6 +919,9 @@ __device__ __forceinline__ void simulate_forward(int inst, const BatchArgs& a, c
   }
   if (status != AMX_SUCCESS) {
     llh = NAN;
+    if (a.diag && tm.lane == 0)
+      for (int j = it; j < a.nt; ++j)
+        for (int k = 0; k < AMX_NDIAG; ++k) a.diag[(I * a.nt + j) * AMX_NDIAG + k] = 0;
     for (int j = it; j < a.nt; ++j) {
       if (a.x && tm.lane < NX) a.x[(I * a.nt + j) * NX + tm.lane] = NAN;
       if (a.y)
@@ -939,6 +960,10 @@ __device__ __forceinline__ void simulate_backward(int inst, const BatchArgs& a, 
   using PB = BwdProblem<M>;
   constexpr int EPQ = PB::EPQ;
   const size_t I = (size_t)inst;
+  int* const diag = a.diag ? a.diag + I * a.nt * AMX_NDIAG : nullptr;
+  if (diag && tm.lane == 0)
+    for (int j = 0; j < a.nt; ++j)
+      for (int k = 5; k < AMX_NDIAG; ++k) diag[(size_t)j * AMX_NDIAG + k] = 0;
   const int fst = a.status[inst];
   if (fst != AMX_SUCCESS) {
     for (int j = tm.lane; j < NP; j += T) a.sllh[I * NP + j] = NAN;
@@ -963,39 +988,67 @@ __device__ __forceinline__ void simulate_backward(int inst, const BatchArgs& a, 
   double qB[EPQ];
 #pragma unroll
   for (int e = 0; e < EPQ; ++e) qB[e] = 0.0;
+  // per-output-time backward diagnostics (Solver::store_diagnosis_b,
+  // src/solver.cpp:377-398, called from src/backwardproblem.cpp:222-290):
+  // entry k at the k-th data point from the last one, zeros until the
+  // backward solver has been called
+  int ndiag = 0;
   if (it >= 0 && ts[it] == tcur) {
     if (tm.lane < NX) xB += dJ[(size_t)it * NX + tm.lane];
     --it;
+    ++ndiag;
   }
   int status = AMX_SUCCESS;
-  int st0 = 0, st1 = 0, st2 = 0, st3 = 0, st4 = 0, st5 = 0, st6 = 0, st7 = 0;
-#define AMX_ACC_STATS()                                                        \
-  do {                                                                         \
-    st0 += bdf.nst; st1 += bdf.nfe; st2 += bdf.nsetups; st3 += bdf.nje;        \
-    st4 += bdf.nni; st5 += bdf.netf; st6 += bdf.ncfn; st7 += bdf.nfQe;         \
-  } while (0)
+  // EventHandlingBwdSimulator::run (src/backwardproblem.cpp:267-332) with
+  // CVodeB (cvodea.c:1365-1630): the backward integration stops only at the
+  // checkpoint start t0 (CVodeSetStopTime(ck_t0)), so steps may pass a data
+  // point, whose adjoint state is then the Nordsieck interpolant; after the
+  // jump the problem restarts at order 1 (reset_state) and, once it has been
+  // integrated before, takes its first step through reInit_post_process with
+  // the stop time at the next data point.
+  const int mxB = a.spB.maxsteps;
+  bool called = false, force_pp = false;
   bdf.init(tcur, xB, qB);
   while (it >= 0 && status == AMX_SUCCESS) {
     const double tnext = ts[it];
     if (tnext < tcur) {
-      status = bdf.advance(tnext);
+      if (force_pp) {
+        status = bdf.reinit_step(tnext, mxB);
+        force_pp = false;
+        if (status != AMX_SUCCESS) break;
+      }
+      status = bdf.advance(tnext, t0, mxB);
       if (status != AMX_SUCCESS) break;
+      called = true;
       tcur = tnext;
-      xB = bdf.Z(0);
+      xB = bdf.yout;
 #pragma unroll
-      for (int e = 0; e < EPQ; ++e) qB[e] = bdf.ZQ(0, e);
+      for (int e = 0; e < EPQ; ++e) qB[e] = bdf.yQout[e];
     }
+    if (diag && tm.lane == 0 && ndiag < a.nt) {
+      int* d = diag + (size_t)ndiag * AMX_NDIAG;
+      d[5] = called ? bdf.nst : 0;
+      d[6] = called ? bdf.nfe : 0;
+      d[7] = called ? bdf.netf : 0;
+      d[8] = called ? bdf.ncfn : 0;
+    }
+    ++ndiag;
     if (tm.lane < NX) xB += dJ[(size_t)it * NX + tm.lane];
     --it;
-    AMX_ACC_STATS();
-    bdf.init(tcur, xB, qB);
+    if (called) {
+      force_pp = true;
+      bdf.reset_state(tcur, xB, qB);
+    } else {
+      bdf.init(tcur, xB, qB);
+    }
   }
   if (status == AMX_SUCCESS && tcur > t0) {
-    status = bdf.advance(t0);
+    if (force_pp) status = bdf.reinit_step(t0, mxB);
+    if (status == AMX_SUCCESS) status = bdf.advance(t0, t0, mxB);
     if (status == AMX_SUCCESS) {
-      xB = bdf.Z(0);
+      xB = bdf.yout;
 #pragma unroll
-      for (int e = 0; e < EPQ; ++e) qB[e] = bdf.ZQ(0, e);
+      for (int e = 0; e < EPQ; ++e) qB[e] = bdf.yQout[e];
     }
   }
   // pre-equilibrated instance: sllh also gets -xB(t0) . sx0 with the
@@ -1034,8 +1087,6 @@ __device__ __forceinline__ void simulate_backward(int inst, const BatchArgs& a, 
       }
     }
   }
-  AMX_ACC_STATS();
-#undef AMX_ACC_STATS
   // sllh = -(xB(t0).sx0 + qB) with sx0 = 0; log10 chain rule on non-zero entries
 #pragma unroll
   for (int e = 0; e < EPQ; ++e) {
@@ -1048,10 +1099,7 @@ __device__ __forceinline__ void simulate_backward(int inst, const BatchArgs& a, 
   }
   if (tm.lane == 0) {
     if (status != AMX_SUCCESS) a.status[inst] = status;
-    if (a.stats) {
-      int* o = a.stats + I * AMX_NSTATS + 8;
-      o[0] = st0; o[1] = st1; o[2] = st2; o[3] = st3; o[4] = st4; o[5] = st5; o[6] = st6; o[7] = st7;
-    }
+    if (a.stats) store_stats(a.stats + I * AMX_NSTATS, bdf, 8);
   }
 }
 

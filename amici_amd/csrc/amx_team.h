@@ -1,11 +1,13 @@
 // Team primitives for gfx950: a "team" integrates one ODE instance.  Two
 // kinds:
-//  * T = 1..64 (a power of two): T consecutive lanes of one 64-wide
-//    wavefront.  Different teams of the same wavefront may diverge, so no
-//    workgroup barrier is used.
+//  * T = 64: one whole 64-wide wavefront.  Teams of one block never wait
+//    for each other, so no workgroup barrier is used.
 //  * T = 128 or 256: the whole workgroup (T / 64 wavefronts, one team per
 //    block), for models with more than 64 states.  Cross-wavefront
 //    reductions and exchanges go through LDS with workgroup barriers.
+// Teams narrower than a wavefront are not provided: their reductions would
+// have to separate the teams sharing a wavefront, and every reduction here
+// is built so that its result is one wave-uniform (scalar) value.
 // All team-level control flow is uniform inside a team.  A translation unit
 // holds one kind of multi-wavefront team: AMX_TEAM_LANES names its size.
 #pragma once
@@ -154,22 +156,14 @@ __device__ __forceinline__ TeamXchg& team_xchg() {
 
 template <int T>
 struct Team {
-  static_assert((T >= 1 && T <= 64 && (T & (T - 1)) == 0) ||
-                    ((T == 128 || T == 256) && T == AMX_TEAM_LANES),
-                "team size");
+  static_assert(T == 64 || ((T == 128 || T == 256) && T == AMX_TEAM_LANES),
+                "team size: one wavefront (64) or one workgroup of 128 / 256 lanes");
   static constexpr int W = T > 64 ? T / 64 : 1;  // wavefronts per team
   int lane;  // lane inside the team, 0..T-1
-  int base;  // wave lane of team lane 0 (single-wavefront teams)
 
   __device__ __forceinline__ Team() {
-    if constexpr (T > 64) {
-      lane = threadIdx.x;
-      base = 0;
-    } else {
-      const int wl = __lane_id();
-      lane = wl & (T - 1);
-      base = wl - lane;
-    }
+    if constexpr (T > 64) lane = threadIdx.x;
+    else lane = __lane_id();
   }
 
   // Multi-wavefront combine: every wavefront's (uniform) partial is written
@@ -191,23 +185,15 @@ struct Team {
   __device__ __forceinline__ double sum(double v) const {
     if constexpr (T > 64) {
       return combine(Wave64::sum(v), [](double a, double b) { return a + b; });
-    } else if constexpr (T == 64) {
-      return Wave64::sum(v);
     } else {
-#pragma unroll
-      for (int o = T / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, T);
-      return v;
+      return Wave64::sum(v);
     }
   }
   __device__ __forceinline__ double max(double v) const {
     if constexpr (T > 64) {
       return combine(Wave64::max(v), [](double a, double b) { return fmax(a, b); });
-    } else if constexpr (T == 64) {
-      return Wave64::max(v);
     } else {
-#pragma unroll
-      for (int o = T / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, T));
-      return v;
+      return Wave64::max(v);
     }
   }
   __device__ __forceinline__ int maxi(int v) const {
@@ -221,15 +207,8 @@ struct Team {
       for (int w = 1; w < W; ++w) r = x.i[w] > r ? x.i[w] : r;
       lds_barrier();
       return r;
-    } else if constexpr (T == 64) {
-      return Wave64::maxi(v);
     } else {
-#pragma unroll
-      for (int o = T / 2; o > 0; o >>= 1) {
-        int u = __shfl_xor(v, o, T);
-        v = u > v ? u : v;
-      }
-      return v;
+      return Wave64::maxi(v);
     }
   }
   // (value, index) maximum; ties resolve to the smaller index.
@@ -254,18 +233,8 @@ struct Team {
       lds_barrier();
       idx = bi;
       return bv;
-    } else if constexpr (T == 64) {
-      return Wave64::argmax(v, lane, idx);
     } else {
-      int i = lane;
-#pragma unroll
-      for (int o = T / 2; o > 0; o >>= 1) {
-        double u = __shfl_xor(v, o, T);
-        int j = __shfl_xor(i, o, T);
-        if (u > v || (u == v && j < i)) { v = u; i = j; }
-      }
-      idx = i;
-      return v;
+      return Wave64::argmax(v, lane, idx);
     }
   }
   // Value of v on team lane src (src uniform within the team).
@@ -277,10 +246,8 @@ struct Team {
       const double r = x.d[0];
       lds_barrier();
       return r;
-    } else if constexpr (T == 64) {
-      return readlane_d(v, src);
     } else {
-      return __shfl(v, src, T);
+      return readlane_d(v, src);
     }
   }
   __device__ __forceinline__ int bcast_i(int v, int src) const {
@@ -291,10 +258,8 @@ struct Team {
       const int r = x.i[0];
       lds_barrier();
       return r;
-    } else if constexpr (T == 64) {
-      return __builtin_amdgcn_readlane(v, src);
     } else {
-      return __shfl(v, src, T);
+      return __builtin_amdgcn_readlane(v, src);
     }
   }
   // Value of v on team lane src (src may differ per lane, 0 <= src < T).

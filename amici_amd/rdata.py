@@ -111,13 +111,18 @@ def derived_outputs(y: torch.Tensor, sy: torch.Tensor | None, meas: torch.Tensor
     """res [nb, k*nt*ny], sres [nb, k*nt*ny, np] (k = 2 with error residuals),
     chi2 [nb], FIM [nb, np, np] and ssigmay [nb, nt, np, ny], in ReturnData's
     layout, with failed instances invalidated as ReturnData::invalidate does:
-    res / sres NaN from the first time point not reached, chi2 and FIM NaN."""
+    res / sres NaN from the first time point not reached, chi2 and FIM NaN
+    when the forward problem failed (not after a backward-only failure)."""
     fields = fields if fields is not None else {"res", "sres", "chi2", "FIM", "ssigmay"}
     nb, nt, ny = y.shape
     out = {}
     it0 = first_invalid_time(y, status)                              # [nb]
     tmask = torch.arange(nt, device=y.device)[None, :] >= it0[:, None]  # [nb, nt] invalid
-    failed = status != 0
+    # a failure of the forward problem (or of the pre-equilibration) leaves
+    # output times unreached; a failure of the backward problem only
+    # invalidates sllh (ReturnData::process_simulation_objects,
+    # src/rdata.cpp:251-254), so llh, chi2 and the forward outputs stay
+    failed = (status != 0) & (it0 < nt)
     nanf = torch.tensor(float("nan"), dtype=y.dtype, device=y.device)
 
     res = residuals(y, meas, sigma, noise)

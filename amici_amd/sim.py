@@ -172,11 +172,18 @@ class Model:
         finally:
             solver.sensi_meth, solver.sensi = old
 
-    def workspace(self, nbytes: int, device) -> torch.Tensor:
-        ws = self._ws.get(device)
+    def workspace(self, nbytes: int, device, stream) -> torch.Tensor:
+        """Device workspace of the calls on ``stream`` (work-queue counters,
+        adjoint trajectory): one per stream, so calls in flight on different
+        streams never share one, while calls on one stream run in order.  The
+        buffer is allocated on that stream, so the caching allocator orders
+        its reuse after them."""
+        key = (str(device), stream.cuda_stream)
+        ws = self._ws.get(key)
         if ws is None or ws.numel() < nbytes:
-            ws = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=device)
-            self._ws[device] = ws
+            with torch.cuda.stream(stream):
+                ws = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=device)
+            self._ws[key] = ws
         return ws
 
 
@@ -203,6 +210,7 @@ class Solver:
     ss_rtol: float = math.nan
     ss_atol: float = math.nan
     rdata_reporting: RDataReporting = RDataReporting.full
+    max_blocks: int = 0  # launch shape: 0 = as many blocks as fit the device at once
 
     def set_relative_tolerance(self, v): self.rtol = float(v)
     def get_relative_tolerance(self): return self.rtol
@@ -230,6 +238,14 @@ class Solver:
     def set_sensitivity_method(self, v): self.sensi_meth = SensitivityMethod(int(v))
     def get_sensitivity_method(self): return self.sensi_meth
 
+    def set_max_blocks(self, v):
+        """Caps the blocks of the persistent kernels (no counterpart in the
+        reference): with fewer blocks than instances each block takes further
+        instances from the device work queue.  0 restores the default."""
+        if int(v) < 0:
+            raise ValueError("max_blocks must be >= 0")
+        self.max_blocks = int(v)
+
     def set_linear_solver(self, v):
         if str(v).lower() not in ("dense", "linearsolver.dense", "1"):
             raise ValueError("this backend uses the dense direct linear solver")
@@ -246,7 +262,8 @@ class Solver:
                               quad_rtol=self.quad_rtol, quad_atol=self.quad_atol,
                               maxstepsB=self.maxstepsB, max_traj=self.max_traj,
                               maxtime=self.maxtime, newton_maxsteps=self.newton_maxsteps,
-                              ss_rtol=self.ss_rtol, ss_atol=self.ss_atol)
+                              ss_rtol=self.ss_rtol, ss_atol=self.ss_atol,
+                              max_blocks=self.max_blocks)
 
 
 class ExpData:
@@ -336,14 +353,29 @@ class ReturnData:
     FIM: np.ndarray | None = None
     sigma_res: bool = False
     status: int = AMICI_NOT_RUN
-    numsteps: int = 0
-    numrhsevals: int = 0
-    numstepsB: int = 0
+    # per-output-time solver diagnostics (include/amici/rdata.h:261-301):
+    # cumulative forward counters at each output time, and the backward
+    # problem's cumulative counters in the order the backward sweep stores
+    # them (entry k at the k-th data point from the last one)
+    numsteps: np.ndarray | None = None
+    num_rhs_evals: np.ndarray | None = None
+    num_err_test_fails: np.ndarray | None = None
+    num_non_lin_solv_conv_fails: np.ndarray | None = None
+    order: np.ndarray | None = None
+    numsteps_b: np.ndarray | None = None
+    num_rhs_evals_b: np.ndarray | None = None
+    num_err_test_fails_b: np.ndarray | None = None
+    num_non_lin_solv_conv_fails_b: np.ndarray | None = None
     stats: np.ndarray = field(default_factory=lambda: np.zeros(_lib.AMX_NSTATS, np.int64))
     messages: list = field(default_factory=list)
 
     def __getitem__(self, key):
         return getattr(self, key)
+
+
+DIAG_FIELDS = ("numsteps", "num_rhs_evals", "num_err_test_fails", "num_non_lin_solv_conv_fails",
+               "order", "numsteps_b", "num_rhs_evals_b", "num_err_test_fails_b",
+               "num_non_lin_solv_conv_fails_b")
 
 
 def _as_dev(a, dev, dtype=torch.float64) -> torch.Tensor:
@@ -402,13 +434,16 @@ def simulate_batch(model: Model, solver: Solver, theta, kfix, ts, meas=None, sig
         "status": torch.full((nb,), AMICI_NOT_RUN, dtype=torch.int32, device=dev),
         "stats": (torch.zeros((nb, _lib.AMX_NSTATS), dtype=torch.int32, device=dev)
                   if want_stats else None),
+        "diag": (torch.zeros((nb, nt, _lib.AMX_NDIAG), dtype=torch.int32, device=dev)
+                 if want_stats else None),
     }
     bin_ = _lib.BatchIn(nb, nt, ts_stride, _ptr(theta_t), _ptr(kfix_t) if model.nk else None,
                         _ptr(ts_t), _ptr(meas_t), _ptr(sigma_t), _ptr(kpre_t))
-    bout = _lib.BatchOut(*[_ptr(out[k]) for k in ("x", "y", "sx", "llh", "sllh", "status", "stats")])
+    bout = _lib.BatchOut(*[_ptr(out[k]) for k in ("x", "y", "sx", "llh", "sllh", "status", "stats",
+                                                  "diag")])
     nbytes = lib.workspace_bytes(opts, 1 if sequential else nb, nt)
-    ws = model.workspace(nbytes, dev)
     st = stream if stream is not None else torch.cuda.current_stream(dev)
+    ws = model.workspace(nbytes, dev, st)
     fn = lib.lib.amx_run_sequential if sequential else lib.lib.amx_run_batch
     import ctypes
     rc = fn(ctypes.byref(opts), ctypes.byref(bin_), ctypes.byref(bout), ws.data_ptr(), ws.numel(),
@@ -427,8 +462,6 @@ def simulate_concurrent(groups: Sequence[dict]) -> list:
     share the GPU; inputs are staged and outputs allocated on the group's
     stream.  Waits for all groups and returns their result dicts in order.
     Results equal those of running the groups one after another."""
-    if len({id(g["model"]) for g in groups}) != len(groups):
-        raise ValueError("one group per Model object (each model owns one workspace)")
     dev = _device()
     streams = [torch.cuda.Stream(device=dev) for _ in groups]
     cur = torch.cuda.current_stream(dev)
@@ -505,9 +538,10 @@ def run_simulations(model: Model, solver: Solver, edata_list: Sequence[ExpData],
     res = []
     for b, e in enumerate(edata_list):
         st = host["stats"][b]
+        dg = host["diag"][b].astype(np.int64)
         rd = ReturnData(ts=np.asarray(e.timepoints), status=int(host["status"][b]),
-                        numsteps=int(st[0]), numrhsevals=int(st[1]), numstepsB=int(st[8]),
-                        stats=st.astype(np.int64), sigma_res=model.get_add_sigma_residuals())
+                        stats=st.astype(np.int64), sigma_res=model.get_add_sigma_residuals(),
+                        **{k: dg[:, i].copy() for i, k in enumerate(DIAG_FIELDS)})
         if "x" in fields:
             rd.x = host["x"][b]
         if "y" in fields:

@@ -41,6 +41,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_cores() -> tuple:
+    """Threads for the CPU baseline: every core this process may run on
+    (its affinity set), limited by the CPU share the machine grants one GPU
+    job when it states one in OMP_NUM_THREADS.  Returns (threads, note)."""
+    aff = len(os.sched_getaffinity(0))
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    n = aff
+    if share.isdigit() and 0 < int(share) < aff:
+        n = int(share)
+    note = (f"{n} threads: affinity set {aff} cores, os.cpu_count() {os.cpu_count()}, "
+            f"OMP_NUM_THREADS {share or 'unset'} (the CPU share of this GPU job)")
+    return n, note
+
+
 def flop_model(counters: np.ndarray, n: int, ops: dict, nt: int, ny: int) -> tuple:
     """Algorithmic FLOPs per instance of the forward and of the backward kernel,
     from CVODES counters at rtol 1e-8 (dense-equivalent LU and solve costs):
@@ -93,13 +107,14 @@ def cpu_baseline(cs, theta, kfix, meas, sigma, sample: int, config: str):
     except Exception as e:  # library not built on this machine
         log(f"cpu baseline skipped: {e}")
         return None
-    cores = min(16, len(os.sched_getaffinity(0)))
+    cores, note = host_cores()
     t0 = time.perf_counter()
     r = O.simulate(cs["network"], theta[:sample], kfix[:sample], cs["ts"], meas[:sample],
                    sigma[:sample], sensi=2, nthreads=cores, **TOL)
     dt = time.perf_counter() - t0
     ok = int((r["status"] == 0).sum())
-    return {"value": sample / dt, "unit": "evals/s", "cores": cores, "kind": "reference",
+    return {"value": ok / dt, "unit": "evals/s", "cores": cores, "cores_note": note,
+            "kind": "reference",
             "sample": f"{sample} of {theta.shape[0]} {config} conditions, adjoint NLL+gradient, "
                       f"CVODES 7.1.1 "
                       f"(reference sources) + AMICI layer restated in C, OpenMP over conditions, "
@@ -197,10 +212,11 @@ def main() -> int:
     model.lib.set_timing(False)
     status = res["status"].cpu().numpy()
     sllh = res["sllh"].cpu().numpy()
-    n_ok = int((status == 0).sum())
-    if n_ok != nb or not np.isfinite(sllh).all():
+    n_ok = int(((status == 0) & np.isfinite(sllh).all(axis=1)).sum())
+    if n_ok != nb:
         log(f"warning: {nb - n_ok} instances failed: {np.unique(status, return_counts=True)}")
-    value = nb * args.steps / dt
+    # only successful evaluations count
+    value = n_ok * args.steps / dt
 
     # solver counters of this path (for the record) and the CVODES counters
     # that define the algorithmic work
@@ -222,8 +238,13 @@ def main() -> int:
     dom_ms = bwd_ms if dom == "bwd" else fwd_ms
     achieved = dom_flops / (dom_ms * 1e-3) / 1e12
     tr = traffic_from_profile(cfg["traffic"])
+    # the compute roofline: MI355X's FP64 peak (78.6 TFLOP/s for the vector
+    # units and for the matrix cores alike); what limits the kernel is stated
+    # beside it
     roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
+                "issued": "FP64 VALU (this kernel issues no MFMA instruction)",
+                "limiter": "per-instance dependent-instruction latency (one wavefront per SIMD)",
                 "traffic": (tr or {}).get("bytes_per_launch") if tr else None,
                 "kernel": "amx_bwd_adj" if dom == "bwd" else "amx_fwd_adj",
                 "kernel_ms": dom_ms, "flops_per_launch": dom_flops,
@@ -239,6 +260,7 @@ def main() -> int:
 
     cpu = cpu_baseline(cs, cs["theta"], cs["kfix"], meas_np, sigma_np, args.cpu_sample,
                        args.config) if args.cpu_sample > 0 else None
+    accuracy = accuracy_gate(model, adj, dev) if args.config == "N40" else None
 
     line = {
         "metric": METRIC, "value": value, "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
@@ -254,11 +276,53 @@ def main() -> int:
         "solver_steps_mean": {"fwd": float(stats[:, 0].mean()), "bwd": float(stats[:, 8].mean()),
                               "cvodes_fwd": float(gold[:, 0].mean()),
                               "cvodes_bwd": float(gold[:, 9].mean())},
+        "accuracy": accuracy,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def _rel(a, ref, floor):
+    """Largest |a - ref| / (|ref| + floor * max|ref| of the instance)."""
+    scale = np.abs(ref).reshape(ref.shape[0], -1).max(axis=1)
+    scale = scale.reshape((-1,) + (1,) * (ref.ndim - 1))
+    return float(np.max(np.abs(a - ref) / (np.abs(ref) + floor * scale)))
+
+
+def _normwise(a, ref):
+    """Per instance max|a - ref| / max|ref| (the gradient as a vector)."""
+    return np.abs(a - ref).max(axis=1) / np.abs(ref).max(axis=1)
+
+
+def accuracy_gate(model, solver, dev):
+    """The timed solver settings (rtol 1e-8 / atol 1e-12) on the 16 N40
+    conditions of the oracle fixtures: errors against the oracle at rtol
+    1e-10 beside the oracle's own errors at the timed tolerances."""
+    from amici_amd import sim
+
+    g = np.load(ROOT / "tests" / "golden" / "n40_conditions16.npz")
+    c = np.load(ROOT / "tests" / "golden" / "n40_conditions16_rtol8.npz")
+    out = sim.simulate_batch(model, solver, g["theta"], g["kfix"], g["ts"], g["meas"], g["sigma"])
+    import torch
+    torch.cuda.synchronize()
+    st = out["status"].cpu().numpy()
+    x, y = out["x"].cpu().numpy(), out["y"].cpu().numpy()
+    llh, sllh = out["llh"].cpu().numpy(), out["sllh"].cpu().numpy()
+    gpu_nw, cv_nw = _normwise(sllh, g["sllh_asa"]), _normwise(c["sllh_asa"], g["sllh_asa"])
+    return {
+        "conditions": int(len(st)), "succeeded": int((st == 0).sum()),
+        "reference": "oracle (CVODES 7.1.1) at rtol 1e-10 / atol 1e-14",
+        "gpu": {"x_rel": _rel(x, g["x"], 1e-9), "y_rel": _rel(y, g["y"], 1e-9),
+                "llh_rel": float(np.max(np.abs(llh - g["llh"]) / np.abs(g["llh"]))),
+                "sllh_rel_entrywise": _rel(sllh, g["sllh_asa"], 1e-7),
+                "sllh_rel_normwise": float(gpu_nw.max())},
+        "cvodes_rtol8": {"x_rel": _rel(c["x"], g["x"], 1e-9), "y_rel": _rel(c["y"], g["y"], 1e-9),
+                         "llh_rel": float(np.max(np.abs(c["llh"] - g["llh"]) / np.abs(g["llh"]))),
+                         "sllh_rel_entrywise": _rel(c["sllh_asa"], g["sllh_asa"], 1e-7),
+                         "sllh_rel_normwise": float(cv_nw.max())},
+    }
 
 
 def run_mix(args, dev) -> int:
@@ -405,12 +469,13 @@ def run_rb(args, dev) -> int:
     try:
         from oracle import cvodes_oracle as O
         O.lib()
-        cores = min(16, len(os.sched_getaffinity(0)))
+        cores, note = host_cores()
         t2 = time.perf_counter()
         r = O.simulate(net, cs["theta"][:ns], cs["kfix"][:ns], cs["ts"], meas_np[:ns], sigma_np[:ns],
                        sensi=1, nthreads=cores, **TOL)
         cdt = time.perf_counter() - t2
-        cpu = {"value": ns / cdt, "unit": "evals/s", "cores": cores, "kind": "reference",
+        cpu = {"value": int((r["status"] == 0).sum()) / cdt, "unit": "evals/s", "cores": cores,
+               "cores_note": note, "kind": "reference",
                "sample": f"{ns} of {nb} RB parameter sets, forward sensitivities, CVODES 7.1.1 "
                          f"(reference sources) + AMICI layer restated in C, {cdt:.2f} s wall, "
                          f"{int((r['status'] == 0).sum())}/{ns} succeeded"}
@@ -445,7 +510,7 @@ def mix_cpu_baseline(groups):
     except Exception as e:  # library not built on this machine
         log(f"cpu baseline skipped: {e}")
         return None
-    cores = min(16, len(os.sched_getaffinity(0)))
+    cores, note = host_cores()
     samples = {"robertson": 128, "n40open": 32, "n200open": 16}
     t_total, parts = 0.0, []
     for g in groups:
@@ -464,7 +529,8 @@ def mix_cpu_baseline(groups):
         t_total += cs["theta"].shape[0] / rate
         parts.append(f"{name} {k} conditions {rate:.1f} evals/s")
     total = sum(g["theta"].shape[0] for g in groups)
-    return {"value": total / t_total, "unit": "evals/s", "cores": cores, "kind": "reference",
+    return {"value": total / t_total, "unit": "evals/s", "cores": cores, "cores_note": note,
+            "kind": "reference",
             "sample": "CVODES 7.1.1 (reference sources) + AMICI layer restated in C, OpenMP over "
                       "conditions, adjoint with pre-equilibration for the open 40-species group "
                       "(the open 200-species group without it: an upper bound on the CPU rate); "

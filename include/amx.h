@@ -26,8 +26,9 @@
 extern "C" {
 #endif
 
-#define AMX_ABI_VERSION 2
+#define AMX_ABI_VERSION 3
 #define AMX_NSTATS 16
+#define AMX_NDIAG 9
 
 /* sensitivity method (amici::SensitivityMethod, include/amici/defines.h) */
 #define AMX_SENSI_NONE 0
@@ -48,6 +49,11 @@ typedef struct {
     int newton_maxsteps;       /* pre-equilibration: damped Newton iterations (Solver::setNewtonMaxSteps) */
     double ss_rtol, ss_atol;   /* steady-state tolerances; NaN -> 100 x rtol / atol
                                   (Solver ss_tol_factor, include/amici/solver.h) */
+    int max_blocks;            /* 0: the persistent kernels get as many blocks as fit on the
+                                  device at once; > 0: at most this many blocks, so fewer
+                                  instances run at a time and each block takes further
+                                  instances from the work queue (no counterpart in the
+                                  reference: a launch-shape control)                    */
 } amx_solver_opts;
 
 typedef struct {
@@ -73,6 +79,16 @@ typedef struct {
     double* sllh;   /* [nb][np] gradient of llh w.r.t. log10 params */
     int* status;    /* [nb] AMICI status code per instance (required) */
     int* stats;     /* [nb][AMX_NSTATS] solver counters or NULL      */
+    int* diag;      /* [nb][nt][AMX_NDIAG] per-output-time diagnostics or NULL
+                       (ReturnData numsteps, num_rhs_evals, num_err_test_fails,
+                       num_non_lin_solv_conv_fails, order, numsteps_b, num_rhs_evals_b,
+                       num_err_test_fails_b, num_non_lin_solv_conv_fails_b,
+                       include/amici/rdata.h:261-301, filled as Solver::store_diagnosis /
+                       store_diagnosis_b do, src/solver.cpp:349-398): slots 0-4 are
+                       cumulative forward counters at output time it; slots 5-8 are
+                       the backward problem's cumulative counters in the order the
+                       backward sweep stores them (entry k at the k-th data point from
+                       the last one), zero without an adjoint run */
 } amx_batch_out;
 
 /* dims[0..7] = nx, nr, np, nk, ny, team lanes per instance, noise model, ABI version */
@@ -83,7 +99,11 @@ size_t amx_workspace_bytes(const amx_solver_opts* opts, int nb, int nt);
 
 /* Runs the whole batch on `stream` (a hipStream_t; NULL = default stream).
  * Returns 0 when the launches were issued; per-instance results in
- * out->status after the stream completes.  Error codes:
+ * out->status after the stream completes.  Re-entrant: calls may run
+ * concurrently from several threads or streams as long as each call in flight
+ * has its own workspace (the work-queue counters and the adjoint trajectory
+ * live there); calls sharing a workspace must be ordered on one stream.
+ * Error codes:
  *   -22 invalid argument, -23 workspace too small, -30 HIP launch error.
  * Pre-equilibration failures are reported per instance as
  * AMICI_NO_STEADY_STATE (-81), AMICI_DAMPING_FACTOR_ERROR (-86) or
@@ -101,15 +121,15 @@ int amx_run_sequential(const amx_solver_opts* opts, const amx_batch_in* in, amx_
  * memory, `capacity` records of 8 doubles: event (1 step, 2 nonlinear-solver
  * failure, 3 error-test failure), t, h, order, error norm or flag, cumulative
  * Newton iterations, linear-solver setups, Jacobian evaluations).  Applies to
- * the launches that follow; buffer NULL switches tracing off.  Process-wide
- * setting, not thread-safe. */
+ * the launches that follow, from any thread (a process-wide setting; the
+ * library serialises access to it). */
 int amx_set_trace(double* buffer, int instance, int capacity);
 
 /* Benchmark instrumentation (no counterpart in the reference): while enabled,
  * every amx_run_batch / amx_run_sequential launch records HIP events on its
  * stream around its forward kernel and, for the adjoint, its backward kernel
  * (up to 4096 launches between reads).  Not for use under graph capture.
- * Process-wide setting, not thread-safe. */
+ * A process-wide setting; the library serialises access to the records. */
 int amx_set_timing(int enable);
 
 /* Waits for the recorded launches and returns the summed kernel durations

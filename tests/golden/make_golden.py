@@ -115,6 +115,8 @@ def main() -> None:
                         meas=meas_l, sigma=sig_l, x=f["x"], y=f["y"], sx=f["sx"], llh=f["llh"],
                         sllh_fsa=f["sllh"], sllh_asa=a["sllh"])
     n40_counters()
+    n40_bench_accuracy()
+    robertson_mix()
     n40_preeq()
     for p in sorted(OUT.glob("*.npz")):
         print(p.name, p.stat().st_size)
@@ -137,6 +139,41 @@ def n40_preeq() -> None:
     np.savez_compressed(OUT / "n40open_preeq16.npz", theta=theta, kfix=kfix, kfix_preeq=kpre,
                         ts=ts, meas=meas, sigma=sigma, x=f["x"], y=f["y"], llh=f["llh"],
                         sllh_fsa=f["sllh"], sllh_asa=a["sllh"], sx0=f["sx"][:2, 0])
+
+
+def n40_bench_accuracy() -> None:
+    """The 16 N40 conditions of n40_conditions16.npz solved by the oracle at
+    the benchmark's tolerances (rtol 1e-8 / atol 1e-12): CVODES's own error
+    at those settings, against which the timed path's error is gated."""
+    g = np.load(OUT / "n40_conditions16.npz")
+    net = networks.network_n40()
+    args = (net, g["theta"], g["kfix"], g["ts"], g["meas"], g["sigma"])
+    f = run(*args, 1, BENCH)
+    a = run(*args, 2, BENCH)
+    np.savez_compressed(OUT / "n40_conditions16_rtol8.npz", x=a["x"], y=a["y"], llh=a["llh"],
+                        sllh_asa=a["sllh"], sllh_fsa=f["sllh"])
+
+
+def robertson_mix() -> None:
+    """The 1024 Robertson conditions of the mixed workload (parameter sets of
+    seed 5, log10 k ~ N(nominal, 0.5)): data are each set's own observables
+    (oracle, rtol 1e-10) with N(0, 0.01^2) noise (seed 5).  Adjoint llh and
+    sllh at rtol 1e-10 / atol 1e-14 (where the oracle's backward problem of
+    one condition fails its error test: status kept) and at the benchmark's
+    rtol 1e-8 / atol 1e-12 (all succeed), plus the rtol-1e-8 solver
+    counters."""
+    net = networks.robertson()
+    g = networks.mix_groups()[0]
+    theta, kfix, ts = g["theta"], g["kfix"], g["ts"]
+    y = O.simulate(net, theta, kfix, ts, nthreads=8, **TIGHT)["y"]
+    meas = networks.add_noise(y, net.noise, g["sigma"], seed=g["noise_seed"])
+    sigma = np.full_like(meas, g["sigma"])
+    t = O.simulate(net, theta, kfix, ts, meas, sigma, sensi=2, nthreads=8, **TIGHT)
+    b = run(net, theta, kfix, ts, meas, sigma, 2, BENCH)
+    np.savez_compressed(OUT / "robertson_mix1024.npz", theta=theta, ts=ts, meas=meas,
+                        sigma=g["sigma"], llh=t["llh"], sllh_asa=t["sllh"], status=t["status"],
+                        llh_rtol8=b["llh"], sllh_asa_rtol8=b["sllh"],
+                        counters_rtol8=b["counters"].astype(np.int32))
 
 
 def n40_counters() -> None:
@@ -208,6 +245,10 @@ def step_limit() -> None:
 if __name__ == "__main__":
     if sys.argv[1:] == ["n40_counters"]:
         n40_counters()
+    elif sys.argv[1:] == ["n40_bench_accuracy"]:
+        n40_bench_accuracy()
+    elif sys.argv[1:] == ["robertson_mix"]:
+        robertson_mix()
     elif sys.argv[1:] == ["step_limit"]:
         step_limit()
     elif sys.argv[1:] == ["n200"]:

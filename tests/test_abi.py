@@ -149,3 +149,35 @@ def test_tables_match_sympy():
                       for q in range(net.ny)])
         y_sym = np.array([float(e.subs(subs)) for e in sm["y"]])
         np.testing.assert_allclose(y, y_sym, rtol=1e-14)
+
+
+def test_teams_narrower_than_a_wavefront_are_refused():
+    """A team is one 64-lane wavefront or a workgroup of 128 / 256 lanes;
+    narrower teams (several instances per wavefront) are not provided."""
+    net = networks.robertson()
+    for team in (1, 4, 32):
+        with pytest.raises(ValueError, match="team size"):
+            codegen.emit_model_traits(net, team)
+    with pytest.raises(ValueError, match="team size"):
+        codegen.emit_model_traits(networks.network_n40(), 32)
+
+
+def test_abi_v3_struct_layout():
+    """amx_solver_opts ends with max_blocks and amx_batch_out with diag
+    (include/amx.h, ABI version 3)."""
+    assert _lib.AMX_ABI_VERSION == 3
+    assert _lib.SolverOpts._fields_[-1][0] == "max_blocks"
+    assert _lib.BatchOut._fields_[-1][0] == "diag"
+    text = (ROOT / "include" / "amx.h").read_text()
+    assert "#define AMX_ABI_VERSION 3" in text and "#define AMX_NDIAG 9" in text
+
+
+def test_negative_max_blocks_rejected(lib):
+    net, L = lib
+    buf = (ctypes.c_double * 4)()
+    p = ctypes.cast(buf, ctypes.c_void_p).value
+    ins = _lib.BatchIn(1, 3, 0, p, p if net.nk else None, p, None, p, None)
+    outs = _lib.BatchOut(None, None, None, p, p, p, None, None)
+    o = _lib.make_opts(max_blocks=-1)
+    assert L.lib.amx_run_batch(ctypes.byref(o), ctypes.byref(ins), ctypes.byref(outs),
+                               p, 8, None) == -22

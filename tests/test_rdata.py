@@ -148,6 +148,20 @@ def test_failed_instances_are_invalidated():
     assert torch.isfinite(out["FIM"][0]).all() and torch.isnan(out["FIM"][1:]).all()
 
 
+def test_backward_failure_keeps_forward_outputs():
+    """A failure of the backward problem only (status set, every output time
+    reached) invalidates sllh alone in the reference (src/rdata.cpp:251-254):
+    residuals, chi2 and FIM of the forward outputs stay."""
+    noise = networks.NOISE_NORMAL
+    y0, sy, meas, sigma = _case(noise, missing=False)
+    ok = rdata.derived_outputs(y0, sy, meas, sigma, OK, noise)
+    status = torch.tensor([0, -3, 0], dtype=torch.int32)
+    out = rdata.derived_outputs(y0, sy, meas, sigma, status, noise)
+    assert rdata.first_invalid_time(y0, status).tolist() == [NT, NT, NT]
+    for k in ("res", "sres", "chi2", "FIM"):
+        assert torch.equal(out[k], ok[k]), k
+
+
 def test_reporting_modes():
     F = rdata.RDataReporting
     full = rdata.reported_fields(F.full, forward=True, sensi=True)
