@@ -15,7 +15,8 @@ AMX_NDIAG = 9
 SENSI_NONE, SENSI_FORWARD, SENSI_ADJOINT = 0, 1, 2
 
 SYMBOLS = ("amx_model_info", "amx_workspace_bytes", "amx_run_batch", "amx_run_sequential",
-           "amx_set_trace", "amx_set_timing", "amx_kernel_times", "amx_status_string")
+           "amx_set_trace", "amx_set_timing", "amx_kernel_times", "amx_status_string",
+           "amx_graph_create", "amx_graph_launch", "amx_graph_destroy")
 
 
 class SolverOpts(ctypes.Structure):
@@ -81,6 +82,14 @@ class ModelLib:
         L.amx_set_timing.restype = ctypes.c_int
         L.amx_kernel_times.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
         L.amx_kernel_times.restype = ctypes.c_int
+        L.amx_graph_create.argtypes = [ctypes.POINTER(SolverOpts), ctypes.POINTER(BatchIn),
+                                       ctypes.POINTER(BatchOut), ctypes.c_void_p, ctypes.c_size_t,
+                                       ctypes.POINTER(ctypes.c_void_p)]
+        L.amx_graph_create.restype = ctypes.c_int
+        L.amx_graph_launch.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.amx_graph_launch.restype = ctypes.c_int
+        L.amx_graph_destroy.argtypes = [ctypes.c_void_p]
+        L.amx_graph_destroy.restype = ctypes.c_int
         L.amx_status_string.argtypes = [ctypes.c_int]
         L.amx_status_string.restype = ctypes.c_char_p
         dims = (ctypes.c_int * 8)()

@@ -6,7 +6,9 @@ is the batched adjoint sweep (the gradient is -sllh, sllh from
 src/backwardproblem.cpp:27-131 and src/rdata.cpp:514-583 with the log10 chain
 rule of src/rdata.cpp:782-830).  The forward pass already runs the adjoint, so
 backward costs nothing extra; conditions whose simulation fails give NaN NLL
-and gradient.
+and gradient.  With ``graph=True`` the batch is captured once per shape and
+solver setting as a HIP graph (sim.BatchGraph) and replayed on later calls,
+the new inputs copied into the graph's buffers.
 """
 from __future__ import annotations
 
@@ -17,11 +19,32 @@ import torch
 from . import sim
 
 
+def _graph_for(model, solver, theta, kfix, ts, meas, sigma) -> sim.BatchGraph:
+    """The model's cached graph for this solver setting and input shapes
+    (captured on first use), with the inputs copied in."""
+    key = (repr(sorted(vars(solver).items())), tuple(theta.shape), tuple(torch.as_tensor(ts).shape),
+           tuple(torch.as_tensor(meas).shape))
+    cache = model.__dict__.setdefault("_nll_graphs", {})
+    g = cache.get(key)
+    if g is None:
+        g = sim.BatchGraph(model, solver, theta, kfix, ts, meas, sigma, want_x=False,
+                           want_y=False, want_stats=False)
+        cache[key] = g
+    else:
+        g.set_inputs(theta=theta, kfix=kfix, ts=ts, meas=meas, sigma=sigma)
+    return g
+
+
 class NLL(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, theta, model, solver, kfix, ts, meas, sigma):
-        out = sim.simulate_batch(model, solver, theta.detach().to(torch.float64), kfix, ts, meas,
-                                 sigma, want_x=False, want_y=False, want_stats=False)
+    def forward(ctx, theta, model, solver, kfix, ts, meas, sigma, graph=False):
+        th = theta.detach().to(torch.float64)
+        if graph:
+            out = {k: (v.clone() if v is not None else None)
+                   for k, v in _graph_for(model, solver, th, kfix, ts, meas, sigma).launch().items()}
+        else:
+            out = sim.simulate_batch(model, solver, th, kfix, ts, meas, sigma, want_x=False,
+                                     want_y=False, want_stats=False)
         ok = (out["status"] == 0).unsqueeze(1)
         sllh = torch.where(ok, out["sllh"], torch.full_like(out["sllh"], math.nan))
         ctx.save_for_backward(sllh)
@@ -31,7 +54,7 @@ class NLL(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad):
         (sllh,) = ctx.saved_tensors
-        return -grad.unsqueeze(1) * sllh, None, None, None, None, None, None
+        return -grad.unsqueeze(1) * sllh, None, None, None, None, None, None, None
 
 
 def adjoint_solver(**kw) -> sim.Solver:
@@ -42,13 +65,14 @@ def adjoint_solver(**kw) -> sim.Solver:
 
 
 def nll(theta: torch.Tensor, model: sim.Model, kfix, ts, meas, sigma,
-        solver: sim.Solver | None = None) -> torch.Tensor:
+        solver: sim.Solver | None = None, graph: bool = False) -> torch.Tensor:
     """Per-condition NLL [nb] of log10 parameters ``theta`` [nb, np] (a
-    device tensor); differentiable w.r.t. ``theta``."""
+    device tensor); differentiable w.r.t. ``theta``.  ``graph`` replays a
+    captured HIP graph of the batch (one per shape and solver setting)."""
     if solver is None:
         solver = adjoint_solver()
     elif solver.effective_method() != sim.SensitivityMethod.adjoint:
         raise ValueError("nll needs a solver with first-order adjoint sensitivities")
     if theta.dim() != 2 or theta.shape[1] != model.np:
         raise ValueError(f"theta must be [nb, {model.np}]")
-    return NLL.apply(theta, model, solver, kfix, ts, meas, sigma)
+    return NLL.apply(theta, model, solver, kfix, ts, meas, sigma, graph)

@@ -452,6 +452,21 @@ struct LUBig {
 
 constexpr int TRACE_REC = 8;
 
+// Optional cycle profile (diagnostic builds compiled with -DAMX_PROFILE):
+// shader-clock cycles spent in each part of the integrator, accumulated per
+// instance.  Product builds compile the hooks away.
+enum : int {
+  PF_STEP, PF_PREDICT, PF_COEFF, PF_NEWTON, PF_RHS, PF_LSETUP, PF_SOLVE, PF_ERRTEST,
+  PF_QUAD, PF_COMPLETE, PF_PREPARE, PF_LOOP, PF_REINIT, PF_N = 16
+};
+#ifdef AMX_PROFILE
+#define AMX_PF_BEGIN(v) const unsigned long long v = __builtin_readcyclecounter()
+#define AMX_PF_END(v, k) prof[k] += __builtin_readcyclecounter() - (v)
+#else
+#define AMX_PF_BEGIN(v)
+#define AMX_PF_END(v, k)
+#endif
+
 struct SolverParams {
   double rtol, atol;
   double rtolQ, atolQ;
@@ -507,12 +522,18 @@ struct BDF {
   // optional step trace (team lane 0 writes records of TRACE_REC doubles)
   double* trace;
   int trace_cap, trace_n;
+#ifdef AMX_PROFILE
+  unsigned long long prof[PF_N];
+#endif
 
   __device__ __forceinline__ BDF(P& p, L& lds, const SolverParams& s) : pb(p), S(lds), sp(s) {
     active = tm.lane < NX;
     trace = nullptr;
     trace_cap = 0;
     trace_n = 0;
+#ifdef AMX_PROFILE
+    for (int k = 0; k < PF_N; ++k) prof[k] = 0;
+#endif
   }
 
   // ---------------- LDS accessors ----------------
@@ -844,6 +865,7 @@ struct BDF {
   // ---------------- linear system ----------------
   // cvLsSetup + cvNlsLSetup; returns NLS_OK or NLS_RECOVER (singular).
   __device__ __forceinline__ int lsetup(int cf) {
+    AMX_PF_BEGIN(t0);
     const double dgamma = fabs(gamma / gammap - 1.0);
     const bool jbad = (nst == 0) || (nst >= nstlj + MSBJ) ||
                       (cf == FAIL_BAD_J && dgamma < DGMAX_JBAD) || (cf == FAIL_OTHER);
@@ -870,14 +892,17 @@ struct BDF {
     crate = 1.0;
     crateS = 1.0;
     nstlp = nst;
+    AMX_PF_END(t0, PF_LSETUP);
     return ok ? NLS_OK : NLS_RECOVER;
   }
 
-  __device__ __forceinline__ double lsolve(double b) const {
+  __device__ __forceinline__ double lsolve(double b) {
+    AMX_PF_BEGIN(t0);
     double x;
     if constexpr (GLU) x = lu.solve(b, tm, pb.mat, S);
     else x = lu.solve(b, tm, S);
     if (gamrat != 1.0) x *= 2.0 / (1.0 + gamrat);
+    AMX_PF_END(t0, PF_SOLVE);
     return x;
   }
 
@@ -885,7 +910,9 @@ struct BDF {
   // cvNlsResidual: y = zn0 + acor, ftemp = f(tn, y), returns residual.
   __device__ __forceinline__ double residual() {
     y = Z(0) + acor;
+    AMX_PF_BEGIN(t0);
     ftemp = pb.rhs(tn, y);
+    AMX_PF_END(t0, PF_RHS);
     nfe++;
     return rl1 * Z(1) + acor - gamma * ftemp;
   }
@@ -1181,22 +1208,34 @@ struct BDF {
     int attempt = 0;
     for (;; ++attempt) {
       if (attempt >= MAX_ATTEMPTS) return AMX_CONV_FAILURE;
+      AMX_PF_BEGIN(t0);
       predict();
+      AMX_PF_END(t0, PF_PREDICT);
+      AMX_PF_BEGIN(t1);
       set_coeffs();
+      AMX_PF_END(t1, PF_COEFF);
+      AMX_PF_BEGIN(t2);
       nflag = nls(nflag);
+      AMX_PF_END(t2, PF_NEWTON);
       int kflag = handle_nflag(nflag, saved_t, ncf);
       if (kflag == PREDICT_AGAIN) continue;
       if (kflag != DO_ERROR_TEST) return kflag;
+      AMX_PF_BEGIN(t3);
       int eflag = error_test(nflag, saved_t, acnrm, nef);
+      AMX_PF_END(t3, PF_ERRTEST);
       if (eflag == TRY_AGAIN) continue;
       if (eflag != AMX_SUCCESS) return eflag;
       dsm = dsm_test;
       if constexpr (QUAD) {
         ncf = nef = 0;
+        AMX_PF_BEGIN(t4);
         quad_correct();
+        AMX_PF_END(t4, PF_QUAD);
         if (quad_errcon()) {
+          AMX_PF_BEGIN(t5);
           const double acnrmQ = wrmsQ(acorQ, ewtQ);
           eflag = error_test(nflag, saved_t, acnrmQ, nefQ);
+          AMX_PF_END(t5, PF_ERRTEST);
           if (eflag == TRY_AGAIN) continue;
           if (eflag != AMX_SUCCESS) return eflag;
           dsm = fmax(dsm, dsm_test);
@@ -1219,9 +1258,13 @@ struct BDF {
       }
       break;
     }
+    AMX_PF_BEGIN(t6);
     complete_step();
+    AMX_PF_END(t6, PF_COMPLETE);
     record(1, dsm);
+    AMX_PF_BEGIN(t7);
     prepare_next_step(dsm);
+    AMX_PF_END(t7, PF_PREPARE);
     etamax = (nst <= SMALL_NST) ? ETA_MAX_ES : ETA_MAX_GS;
     dsm_last = dsm;
     return AMX_SUCCESS;
@@ -1406,6 +1449,7 @@ struct BDF {
       }
     }
     for (int nstloc = 0;; ++nstloc) {
+      AMX_PF_BEGIN(tl);
       if (nst > 0) {
         set_ewt();
         if constexpr (QUAD) if (quad_errcon()) set_ewtQ();
@@ -1426,7 +1470,10 @@ struct BDF {
       tolsf = UROUND * nrm;
       if (tolsf > 1.0) return AMX_TOO_MUCH_ACC;
       tolsf = 1.0;
+      AMX_PF_END(tl, PF_LOOP);
+      AMX_PF_BEGIN(ts);
       const int kflag = step();
+      AMX_PF_END(ts, PF_STEP);
       if (kflag != AMX_SUCCESS) return kflag;
       const double troundoff = FUZZ_FACTOR * UROUND * (fabs(tn) + fabs(h));
       if (fabs(tn - tstop) <= troundoff) tn = tstop;

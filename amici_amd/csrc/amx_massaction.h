@@ -53,6 +53,7 @@ struct BatchArgs {
   int* counter;            // work-queue counter (zero before launch)
   unsigned long long maxtime_ticks;  // per-instance wall-clock limit (0 = none)
   double* trace;           // optional step trace of one instance [cap][TRACE_REC]
+  unsigned long long* prof;  // cycle profile [nb][2][PF_N] (diagnostic builds only)
   int trace_inst, trace_cap;
   SolverParams sp;         // forward
   SolverParams spB;        // backward (adjoint)
@@ -928,6 +929,10 @@ __device__ __forceinline__ void simulate_forward(int inst, const BatchArgs& a, c
         for (int k = tm.lane; k < NY; k += T) a.y[(I * a.nt + j) * NY + k] = NAN;
     }
   }
+#ifdef AMX_PROFILE
+  if (a.prof && tm.lane == 0)
+    for (int k = 0; k < PF_N; ++k) a.prof[I * 2 * PF_N + k] = bdf.prof[k];
+#endif
   if (tm.lane == 0) {
     a.llh[inst] = llh;
     a.status[inst] = status;
@@ -1097,6 +1102,10 @@ __device__ __forceinline__ void simulate_backward(int inst, const BatchArgs& a, 
       a.sllh[I * NP + j] = (status == AMX_SUCCESS) ? (g != 0.0 ? g * pv * M_LN10 : 0.0) : NAN;
     }
   }
+#ifdef AMX_PROFILE
+  if (a.prof && tm.lane == 0)
+    for (int k = 0; k < PF_N; ++k) a.prof[(I * 2 + 1) * PF_N + k] = bdf.prof[k];
+#endif
   if (tm.lane == 0) {
     if (status != AMX_SUCCESS) a.status[inst] = status;
     if (a.stats) store_stats(a.stats + I * AMX_NSTATS, bdf, 8);

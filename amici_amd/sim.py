@@ -388,31 +388,28 @@ def _ptr(t):
     return None if t is None else t.data_ptr()
 
 
-def simulate_batch(model: Model, solver: Solver, theta, kfix, ts, meas=None, sigma=None, *,
-                   want_x: bool = True, want_y: bool = True, want_sx: bool = False,
-                   want_stats: bool = True, sequential: bool = False, stream=None,
-                   kfix_preeq=None) -> dict:
-    """Simulates ``nb`` instances in one batch; arrays may be numpy or torch.
-
-    theta [nb, np] (log10), kfix [nb, nk], ts [nt] or [nb, nt], meas / sigma
-    [nb, nt, ny].  Returns a dict of device tensors: x, y, sx, llh, sllh,
-    status, stats.  ``sequential`` runs the plain one-instance-per-launch path.
-    ``kfix_preeq`` [nb, nk] pre-equilibrates every instance to steady state
-    with these fixed parameters before the simulation.
-    """
+def _prepare(model: Model, solver: Solver, theta, kfix, ts, meas, sigma, *, want_x, want_y,
+             want_sx, want_stats, kfix_preeq, copy_inputs=False) -> dict:
+    """Device inputs, output tensors and the C structs of one batch call.
+    With ``copy_inputs`` every input is a fresh device copy (for a graph that
+    keeps pointing at them)."""
     dev = _device()
-    lib = model.lib
-    theta_t = _as_dev(theta, dev)
+
+    def dev_t(a):
+        t = _as_dev(a, dev)
+        return t.clone() if copy_inputs else t
+
+    theta_t = dev_t(theta)
     nb = theta_t.shape[0]
-    kfix_t = _as_dev(kfix if kfix is not None else np.zeros((nb, model.nk)), dev)
-    ts_t = _as_dev(ts, dev)
+    kfix_t = dev_t(kfix if kfix is not None else np.zeros((nb, model.nk)))
+    ts_t = dev_t(ts)
     nt = ts_t.shape[-1]
     ts_stride = nt if ts_t.dim() == 2 else 0
     if sigma is None:
         sigma = np.ones((nb, nt, model.ny))
-    sigma_t = _as_dev(sigma, dev)
-    meas_t = _as_dev(meas, dev) if meas is not None else None
-    kpre_t = _as_dev(kfix_preeq, dev) if kfix_preeq is not None else None
+    sigma_t = dev_t(sigma)
+    meas_t = dev_t(meas) if meas is not None else None
+    kpre_t = dev_t(kfix_preeq) if kfix_preeq is not None else None
     if kpre_t is not None and kpre_t.shape != (nb, model.nk):
         raise ValueError("kfix_preeq must be [nb, nk]")
     if theta_t.shape != (nb, model.np) or kfix_t.shape != (nb, model.nk):
@@ -441,17 +438,107 @@ def simulate_batch(model: Model, solver: Solver, theta, kfix, ts, meas=None, sig
                         _ptr(ts_t), _ptr(meas_t), _ptr(sigma_t), _ptr(kpre_t))
     bout = _lib.BatchOut(*[_ptr(out[k]) for k in ("x", "y", "sx", "llh", "sllh", "status", "stats",
                                                   "diag")])
-    nbytes = lib.workspace_bytes(opts, 1 if sequential else nb, nt)
+    inputs = dict(theta=theta_t, kfix=kfix_t, ts=ts_t, meas=meas_t, sigma=sigma_t,
+                  kfix_preeq=kpre_t)
+    return dict(dev=dev, nb=nb, nt=nt, opts=opts, bin=bin_, bout=bout, out=out, inputs=inputs)
+
+
+def simulate_batch(model: Model, solver: Solver, theta, kfix, ts, meas=None, sigma=None, *,
+                   want_x: bool = True, want_y: bool = True, want_sx: bool = False,
+                   want_stats: bool = True, sequential: bool = False, stream=None,
+                   kfix_preeq=None) -> dict:
+    """Simulates ``nb`` instances in one batch; arrays may be numpy or torch.
+
+    theta [nb, np] (log10), kfix [nb, nk], ts [nt] or [nb, nt], meas / sigma
+    [nb, nt, ny].  Returns a dict of device tensors: x, y, sx, llh, sllh,
+    status, stats, diag.  ``sequential`` runs the plain one-instance-per-launch
+    path.  ``kfix_preeq`` [nb, nk] pre-equilibrates every instance to steady
+    state with these fixed parameters before the simulation.
+    """
+    import ctypes
+
+    p = _prepare(model, solver, theta, kfix, ts, meas, sigma, want_x=want_x, want_y=want_y,
+                 want_sx=want_sx, want_stats=want_stats, kfix_preeq=kfix_preeq)
+    lib, dev, opts = model.lib, p["dev"], p["opts"]
+    nbytes = lib.workspace_bytes(opts, 1 if sequential else p["nb"], p["nt"])
     st = stream if stream is not None else torch.cuda.current_stream(dev)
     ws = model.workspace(nbytes, dev, st)
     fn = lib.lib.amx_run_sequential if sequential else lib.lib.amx_run_batch
-    import ctypes
-    rc = fn(ctypes.byref(opts), ctypes.byref(bin_), ctypes.byref(bout), ws.data_ptr(), ws.numel(),
-            ctypes.c_void_p(st.cuda_stream))
+    rc = fn(ctypes.byref(opts), ctypes.byref(p["bin"]), ctypes.byref(p["bout"]), ws.data_ptr(),
+            ws.numel(), ctypes.c_void_p(st.cuda_stream))
     if rc != 0:
         raise RuntimeError(f"amx_run_batch returned {rc}")
-    out["_keepalive"] = (theta_t, kfix_t, ts_t, meas_t, sigma_t, kpre_t)
+    out = p["out"]
+    i = p["inputs"]
+    out["_keepalive"] = (i["theta"], i["kfix"], i["ts"], i["meas"], i["sigma"], i["kfix_preeq"])
     return out
+
+
+class BatchGraph:
+    """One batch captured as a HIP graph (``amx_graph_create``) and replayed
+    with :meth:`launch`: the launch sequence of :func:`simulate_batch` bound to
+    device buffers the graph owns.  Inputs are updated in place with
+    :meth:`set_inputs` (same shapes); every replay overwrites ``out``, the
+    dict of output tensors.  Replaces the per-call host work of
+    run_simulations (src/amici.cpp:235-262) for repeated evaluations of one
+    batch shape, such as an optimiser's objective."""
+
+    def __init__(self, model: Model, solver: Solver, theta, kfix, ts, meas=None, sigma=None, *,
+                 want_x: bool = True, want_y: bool = True, want_sx: bool = False,
+                 want_stats: bool = True, kfix_preeq=None):
+        import ctypes
+
+        p = _prepare(model, solver, theta, kfix, ts, meas, sigma, want_x=want_x, want_y=want_y,
+                     want_sx=want_sx, want_stats=want_stats, kfix_preeq=kfix_preeq,
+                     copy_inputs=True)
+        self.model = model
+        self.inputs = p["inputs"]
+        self.out = p["out"]
+        self._opts, self._bin, self._bout = p["opts"], p["bin"], p["bout"]
+        nbytes = model.lib.workspace_bytes(p["opts"], p["nb"], p["nt"])
+        self._ws = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=p["dev"])
+        torch.cuda.synchronize(p["dev"])  # inputs staged before the capture reads their addresses
+        h = ctypes.c_void_p()
+        rc = model.lib.lib.amx_graph_create(ctypes.byref(self._opts), ctypes.byref(self._bin),
+                                            ctypes.byref(self._bout), self._ws.data_ptr(),
+                                            self._ws.numel(), ctypes.byref(h))
+        if rc != 0:
+            raise RuntimeError(f"amx_graph_create returned {rc}")
+        self._h = h
+
+    def set_inputs(self, **kw) -> None:
+        """Copies new values into the graph's input buffers (theta, kfix, ts,
+        meas, sigma, kfix_preeq), on the current stream."""
+        for k, v in kw.items():
+            dst = self.inputs.get(k)
+            if dst is None:
+                raise ValueError(f"the graph has no input {k!r}")
+            src = v if isinstance(v, torch.Tensor) else torch.as_tensor(np.asarray(v))
+            dst.copy_(src.reshape(dst.shape))
+
+    def launch(self, stream=None) -> dict:
+        """Replays the batch on ``stream`` (default: the current stream) and
+        returns ``out`` (valid once the stream has run it)."""
+        import ctypes
+
+        if self._h is None:
+            raise RuntimeError("the graph was closed")
+        st = stream if stream is not None else torch.cuda.current_stream()
+        rc = self.model.lib.lib.amx_graph_launch(self._h, ctypes.c_void_p(st.cuda_stream))
+        if rc != 0:
+            raise RuntimeError(f"amx_graph_launch returned {rc}")
+        return self.out
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None:
+            self.model.lib.lib.amx_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def simulate_concurrent(groups: Sequence[dict]) -> list:

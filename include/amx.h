@@ -111,6 +111,20 @@ size_t amx_workspace_bytes(const amx_solver_opts* opts, int nb, int nt);
 int amx_run_batch(const amx_solver_opts* opts, const amx_batch_in* in, amx_batch_out* out,
                   void* workspace, size_t workspace_bytes, void* stream);
 
+/* Captured launch sequence (no counterpart in the reference: it replaces the
+ * host loop over conditions of run_simulations, src/amici.cpp:235-262, by one
+ * replayable HIP graph).  amx_graph_create records what amx_run_batch would
+ * launch for these arguments (work-queue reset, forward kernel, backward
+ * kernel) into a graph bound to these very buffers; amx_graph_launch replays
+ * it on `stream`.  Inputs may be rewritten in place between replays; results
+ * land in the same output buffers.  The buffers and the workspace must
+ * outlive the graph.  Returns 0, or the error codes of amx_run_batch. */
+typedef struct amx_graph amx_graph;
+int amx_graph_create(const amx_solver_opts* opts, const amx_batch_in* in, amx_batch_out* out,
+                     void* workspace, size_t workspace_bytes, amx_graph** graph);
+int amx_graph_launch(amx_graph* graph, void* stream);
+int amx_graph_destroy(amx_graph* graph);
+
 /* Plain GPU BDF baseline: the same integrator, one instance per kernel launch
  * (sequential launches on `stream`), forward sensitivities or none. */
 int amx_run_sequential(const amx_solver_opts* opts, const amx_batch_in* in, amx_batch_out* out,

@@ -15,9 +15,10 @@ TIGHT = dict(rtol=1e-10, atol=1e-14, quad_rtol=1e-10, quad_atol=1e-14, maxsteps=
              maxtime=60.0)
 
 
+@pytest.mark.parametrize("graph", [False, True])
 @pytest.mark.parametrize("fixture,netf", [("robertson_batch64", networks.robertson),
                                           ("n40_conditions16", networks.network_n40)])
-def test_nll_gradient(gpu, fixture, netf):
+def test_nll_gradient(gpu, fixture, netf, graph):
     import torch
 
     from amici_amd import autograd, sim
@@ -31,7 +32,7 @@ def test_nll_gradient(gpu, fixture, netf):
            [torch.as_tensor(g[k][:nb], device=dev) for k in ("meas", "sigma")]
     solver = autograd.adjoint_solver(**TIGHT)
     theta = torch.as_tensor(g["theta"][:nb], device=dev).clone().requires_grad_(True)
-    f = autograd.nll(theta, model, *args, solver=solver)
+    f = autograd.nll(theta, model, *args, solver=solver, graph=graph)
     assert torch.isfinite(f).all()
     f.sum().backward()
     grad = theta.grad.cpu().numpy()
@@ -47,6 +48,6 @@ def test_nll_gradient(gpu, fixture, netf):
         tm = theta.detach().clone()
         tp[:, j] += eps / 2
         tm[:, j] -= eps / 2
-        fd = ((autograd.nll(tp, model, *args, solver=solver)
-               - autograd.nll(tm, model, *args, solver=solver)) / eps).cpu().numpy()
+        fd = ((autograd.nll(tp, model, *args, solver=solver, graph=graph)
+               - autograd.nll(tm, model, *args, solver=solver, graph=graph)) / eps).cpu().numpy()
         np.testing.assert_allclose(grad[:, j], fd, rtol=1e-3, atol=1e-6 * scale.max())
