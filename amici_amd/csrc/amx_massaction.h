@@ -849,7 +849,7 @@ __device__ __forceinline__ void simulate_forward(int inst, const BatchArgs& a, c
   bdf.init(0.0, x0, nullptr);
   if (a.trace && inst == a.trace_inst) {
     bdf.trace = a.trace;
-    bdf.trace_cap = a.trace_cap;
+    bdf.trace_cap = a.trace_cap / 2;  // second half: the backward problem's records
   }
   if constexpr (MODE == 1) pb.sens_init();
   const double* ts = a.ts + (size_t)a.ts_stride * I;
@@ -984,6 +984,10 @@ __device__ __forceinline__ void simulate_backward(int inst, const BatchArgs& a, 
   if constexpr (BIG) pb0.mat = a.lu_ws + I * (size_t)NX * NX;
   BDF<PB> bdf(pb0, lds, a.spB);  // holds pb0 and the LDS block by reference
   if (a.maxtime_ticks) bdf.sp.deadline = wall_clock64() + a.maxtime_ticks;
+  if (a.trace && inst == a.trace_inst) {
+    bdf.trace = a.trace + (size_t)(a.trace_cap / 2) * TRACE_REC;
+    bdf.trace_cap = a.trace_cap - a.trace_cap / 2;
+  }
   const double* ts = a.ts + (size_t)a.ts_stride * I;
   const double* dJ = a.dJydx + I * a.nt * NX;
   const double t0 = 0.0;

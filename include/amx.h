@@ -131,10 +131,12 @@ int amx_run_sequential(const amx_solver_opts* opts, const amx_batch_in* in, amx_
                        void* workspace, size_t workspace_bytes, void* stream);
 
 /* Diagnostics: record every accepted step and every failed attempt of the
- * forward integration of batch instance `instance` into `buffer` (device
- * memory, `capacity` records of 8 doubles: event (1 step, 2 nonlinear-solver
- * failure, 3 error-test failure), t, h, order, error norm or flag, cumulative
- * Newton iterations, linear-solver setups, Jacobian evaluations).  Applies to
+ * forward integration of batch instance `instance` into the first half of
+ * `buffer` and those of its backward (adjoint) integration into the second
+ * half (device memory, `capacity` records of 8 doubles: event (1 step,
+ * 2 nonlinear-solver failure, 3 error-test failure), t, h, order, error norm
+ * or flag, cumulative Newton iterations, linear-solver setups, Jacobian
+ * evaluations).  Applies to
  * the launches that follow, from any thread (a process-wide setting; the
  * library serialises access to it). */
 int amx_set_trace(double* buffer, int instance, int capacity);

@@ -83,6 +83,9 @@ void amx_oracle_jac(const amx_oracle_model* m, const double* p, const double* x,
                     double* J /* nx*nx row-major */);
 void amx_oracle_dxdotdp(const amx_oracle_model* m, const double* p,
                         const double* x, double* dfdp /* nx*np row-major */);
+int amx_oracle_trace_forward(const amx_oracle_model* m, const amx_oracle_opts* o,
+                             const double* theta, const double* kfix, int nt, const double* ts,
+                             double* rec, int cap);
 double amx_oracle_Jy(int noise, double y, double sigma, double m);
 double amx_oracle_dJydy(int noise, double y, double sigma, double m);
 

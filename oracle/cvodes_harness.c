@@ -644,3 +644,90 @@ int amx_oracle_simulate_batch(const amx_oracle_model* m, const amx_oracle_opts* 
     }
     return nfail;
 }
+
+/* Diagnostics (test infrastructure): the forward integration of one instance
+ * (states only, x0 from kfix, tstop at every output time as in
+ * amx_oracle_simulate) driven in CV_ONE_STEP mode, which takes the same steps
+ * as CV_NORMAL mode and returns after each.  Every accepted step is recorded
+ * as 9 doubles: t, step size used, order used, nst, nfe, nsetups, nje, netf,
+ * ncfn.  Returns the number of records (at most cap) or a negative status. */
+int amx_oracle_trace_forward(const amx_oracle_model* m, const amx_oracle_opts* o,
+                             const double* theta, const double* kfix, int nt, const double* ts,
+                             double* rec, int cap) {
+    const int nx = m->nx, np = m->np;
+    int status = 0, n = 0;
+    SUNContext ctx = NULL;
+    void* cv = NULL;
+    N_Vector x = NULL;
+    SUNMatrix A = NULL;
+    SUNLinearSolver LS = NULL;
+    double* p = (double*)malloc(sizeof(double) * (size_t)(np > 0 ? np : 1));
+    double* work = (double*)calloc((size_t)(3 * m->nr + nx * nx + nx * np + nx), sizeof(double));
+    ud_t ud;
+    ud.m = m;
+    ud.p = p;
+    ud.w = work;
+    ud.mono = work + m->nr;
+    ud.u = work + 2 * m->nr;
+    ud.J = work + 3 * m->nr;
+    ud.dfdp = ud.J + nx * nx;
+    for (int j = 0; j < np; ++j) p[j] = pow(10.0, theta[j]);
+    if (SUNContext_Create(SUN_COMM_NULL, &ctx) != 0) { status = AMICI_ERROR; goto cleanup; }
+    x = N_VNew_Serial(nx, ctx);
+    {
+        double* xd = N_VGetArrayPointer(x);
+        for (int i = 0; i < nx; ++i) xd[i] = (m->x0k[i] >= 0) ? kfix[m->x0k[i]] : m->x0c[i];
+    }
+    cv = CVodeCreate(CV_BDF, ctx);
+    CHECK(CVodeInit(cv, cb_f, 0.0, x));
+    CHECK(CVodeSStolerances(cv, o->rtol, o->atol));
+    CHECK(CVodeSetUserData(cv, &ud));
+    CHECK(CVodeSetStabLimDet(cv, o->stldet ? SUNTRUE : SUNFALSE));
+    A = SUNDenseMatrix(nx, nx, ctx);
+    LS = SUNLinSol_Dense(x, A, ctx);
+    CHECK(CVodeSetLinearSolver(cv, LS, A));
+    CHECK(CVodeSetJacFn(cv, cb_jac));
+    CHECK(CVodeSetMaxNonlinIters(cv, 3));
+    CHECK(CVodeSetMaxConvFails(cv, 10));
+    CHECK(CVodeSetMaxNumSteps(cv, o->maxsteps));
+    {
+        double t = 0.0;
+        for (int it = 0; it < nt; ++it) {
+            const double tout = ts[it];
+            if (tout <= 0.0) continue;
+            while (t < tout) {
+                if (next_t_too_close(t, tout)) break;
+                CHECK(CVodeSetStopTime(cv, tout));
+                const int s = CVode(cv, tout, x, &t, CV_ONE_STEP);
+                if (s < 0) { status = s; goto cleanup; }
+                if (n < cap) {
+                    long v;
+                    double hu;
+                    int qu;
+                    double* r = rec + (size_t)n * 9;
+                    CVodeGetCurrentTime(cv, &r[0]);
+                    CVodeGetLastStep(cv, &hu);
+                    CVodeGetLastOrder(cv, &qu);
+                    r[1] = hu;
+                    r[2] = qu;
+                    CVodeGetNumSteps(cv, &v); r[3] = (double)v;
+                    CVodeGetNumRhsEvals(cv, &v); r[4] = (double)v;
+                    CVodeGetNumLinSolvSetups(cv, &v); r[5] = (double)v;
+                    CVodeGetNumJacEvals(cv, &v); r[6] = (double)v;
+                    CVodeGetNumErrTestFails(cv, &v); r[7] = (double)v;
+                    CVodeGetNumNonlinSolvConvFails(cv, &v); r[8] = (double)v;
+                }
+                ++n;
+            }
+        }
+    }
+cleanup:
+    if (cv) CVodeFree(&cv);
+    if (LS) SUNLinSolFree(LS);
+    if (A) SUNMatDestroy(A);
+    if (x) N_VDestroy(x);
+    if (ctx) SUNContext_Free(&ctx);
+    free(p);
+    free(work);
+    return status < 0 ? status : n;
+}

@@ -143,6 +143,24 @@ def simulate(network, theta, kfix, ts, meas=None, sigma=None, *, rtol=1e-10, ato
                 counters=counters)
 
 
+def trace_forward(network, theta, kfix, ts, *, rtol=1e-8, atol=1e-12, stldet=1, maxsteps=100000,
+                  cap=200000) -> np.ndarray:
+    """Accepted steps of one instance's forward integration (states only):
+    rows of t, step size used, order used, nst, nfe, nsetups, nje, netf, ncfn."""
+    om = OracleModel(network)
+    theta = np.ascontiguousarray(theta, np.float64)
+    kfix = np.ascontiguousarray(np.asarray(kfix, np.float64).reshape(network.nk))
+    ts = np.ascontiguousarray(ts, np.float64)
+    rec = np.zeros((cap, 9))
+    opts = _Opts(rtol, atol, 1e-8, 1e-12, maxsteps, 0, CV_SIMULTANEOUS, CV_POLYNOMIAL, stldet, 0.0)
+    L = lib()
+    n = L.amx_oracle_trace_forward(ctypes.byref(om.c), ctypes.byref(opts), _p(theta), _p(kfix),
+                                   ctypes.c_int(len(ts)), _p(ts), _p(rec), ctypes.c_int(cap))
+    if n < 0:
+        raise RuntimeError(f"oracle trace failed ({n})")
+    return rec[:min(n, cap)]
+
+
 def Jy(noise: int, y: float, sigma: float, m: float) -> float:
     return lib().amx_oracle_Jy(noise, y, sigma, m)
 

@@ -263,8 +263,12 @@ def test_concurrent_groups_equal_sequential(sim_models):
         torch.cuda.synchronize()
         for k in ("llh", "sllh", "status", "x"):
             np.testing.assert_array_equal(c[k].cpu().numpy(), alone[k].cpu().numpy(), err_msg=k)
-    with pytest.raises(ValueError):
-        sim.simulate_concurrent([groups[0], groups[0]])
+    # one model on two concurrent streams: each stream has its own workspace
+    # (work-queue counters, trajectory), so both batches equal the sequential one
+    twice = sim.simulate_concurrent([groups[0], groups[0]])
+    for c in twice:
+        for k in ("llh", "sllh", "status", "x"):
+            np.testing.assert_array_equal(c[k].cpu().numpy(), conc[0][k].cpu().numpy(), err_msg=k)
 
 
 def test_model_simulate(sim_models):

@@ -288,3 +288,19 @@ def test_step_budget_counts_over_the_whole_run():
         assert int(r["status"][0]) == st
         assert st == (-99 if m in steps else -1)
         assert bad == int((steps <= m).sum())
+
+
+def test_forward_trace_matches_counters():
+    """The oracle's step trace (CV_ONE_STEP with tstop at every output time)
+    takes the same steps as the CV_NORMAL run whose counters the fixtures
+    hold: same number of steps, function evaluations and setups, ending at the
+    last output time."""
+    net = networks.robertson()
+    ts = networks.robertson_times()
+    th = networks.robertson_parameter_sets(2, seed=9)
+    for b in range(2):
+        tr = O.trace_forward(net, th[b], np.zeros(0), ts, rtol=1e-8, atol=1e-12)
+        c = O.simulate(net, th[b:b + 1], np.zeros((1, 0)), ts, rtol=1e-8, atol=1e-12)["counters"][0]
+        assert len(tr) == c[0] and tr[-1, 3] == c[0]
+        assert tr[-1, 4] == c[1] and tr[-1, 5] == c[2]
+        assert tr[-1, 0] == ts[-1] and np.all(np.diff(tr[:, 0]) > 0)
