@@ -16,7 +16,8 @@ SENSI_NONE, SENSI_FORWARD, SENSI_ADJOINT = 0, 1, 2
 
 SYMBOLS = ("amx_model_info", "amx_workspace_bytes", "amx_run_batch", "amx_run_sequential",
            "amx_set_trace", "amx_set_timing", "amx_kernel_times", "amx_status_string",
-           "amx_graph_create", "amx_graph_launch", "amx_graph_destroy")
+           "amx_graph_create", "amx_graph_launch", "amx_graph_destroy", "amx_run_batch_static",
+           "amx_static_workspace_bytes")
 
 
 class SolverOpts(ctypes.Structure):
@@ -82,6 +83,11 @@ class ModelLib:
         L.amx_set_timing.restype = ctypes.c_int
         L.amx_kernel_times.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
         L.amx_kernel_times.restype = ctypes.c_int
+        L.amx_run_batch_static.argtypes = [ctypes.POINTER(SolverOpts), ctypes.POINTER(BatchIn),
+                                           ctypes.POINTER(BatchOut), ctypes.c_void_p]
+        L.amx_run_batch_static.restype = ctypes.c_int
+        L.amx_static_workspace_bytes.argtypes = []
+        L.amx_static_workspace_bytes.restype = ctypes.c_size_t
         L.amx_graph_create.argtypes = [ctypes.POINTER(SolverOpts), ctypes.POINTER(BatchIn),
                                        ctypes.POINTER(BatchOut), ctypes.c_void_p, ctypes.c_size_t,
                                        ctypes.POINTER(ctypes.c_void_p)]

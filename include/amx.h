@@ -29,6 +29,8 @@ extern "C" {
 #define AMX_ABI_VERSION 3
 #define AMX_NSTATS 16
 #define AMX_NDIAG 9
+/* bytes of the static workspace of amx_run_batch_static, per GPU */
+#define AMX_STATIC_WORKSPACE_BYTES (64u << 20)
 
 /* sensitivity method (amici::SensitivityMethod, include/amici/defines.h) */
 #define AMX_SENSI_NONE 0
@@ -110,6 +112,23 @@ size_t amx_workspace_bytes(const amx_solver_opts* opts, int nb, int nt);
  * AMICI_SINGULAR_JACOBIAN (-9987) (include/amici/defines.h). */
 int amx_run_batch(const amx_solver_opts* opts, const amx_batch_in* in, amx_batch_out* out,
                   void* workspace, size_t workspace_bytes, void* stream);
+
+/* amx_run_batch without any allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  The workspace is a static
+ * device array of AMX_STATIC_WORKSPACE_BYTES in the library, one per GPU,
+ * held until the process ends; the caller still passes inputs and outputs.
+ * The work-queue counters in it are cleared by the launch sequence itself.
+ * Every call on a GPU shares that GPU's array, so calls that overlap on one
+ * GPU (other streams or threads) race: order them on one stream.  Returns the
+ * codes of amx_run_batch, and -24 when the batch needs more workspace than
+ * the array holds (amx_workspace_bytes() > amx_static_workspace_bytes(); a
+ * smaller batch or opts->max_traj fits).  No hipMalloc / hipFree or
+ * allocating library routine is called; the array's device address is looked
+ * up once per GPU (hipGetSymbolAddress) on the first call made there. */
+int amx_run_batch_static(const amx_solver_opts* opts, const amx_batch_in* in, amx_batch_out* out,
+                         void* stream);
+size_t amx_static_workspace_bytes(void);
 
 /* Captured launch sequence (no counterpart in the reference: it replaces the
  * host loop over conditions of run_simulations, src/amici.cpp:235-262, by one
