@@ -483,7 +483,7 @@ struct SolverParams {
 //   void   quad_rhs(double t, double y, double* fq)   fq[EPQ] (if NQ > 0)
 //   void   jac(double t, double y)                    assembles J into its LDS
 //   void   build_M(double gamma, double* a)           a[j] = (I - gamma*J)[lane][j]
-//   bool   on_step(t, zn0, zn1, hscale)               after every accepted step
+//   bool   on_step(t, zn0, zn1, hscale, q_used)       after every accepted step
 //   sensitivity hooks (if NS > 0): see sens_* in the problem
 template <class P>
 struct BDF {
@@ -1429,7 +1429,7 @@ struct BDF {
 #pragma unroll
         for (int e = 0; e < EPQ; ++e) setZQ(1, e, ZQ(1, e) * h);
       if constexpr (SENS) pb.sens_scale_deriv(h);
-      if (!pb.on_step(tn, Z(0), Z(1), hscale)) return AMX_TRAJ_FULL;
+      if (!pb.on_step(tn, Z(0), Z(1), hscale, 0)) return AMX_TRAJ_FULL;
     } else {
       const double troundoff = FUZZ_FACTOR * UROUND * (fabs(tn) + fabs(h));
       if (fabs(tn - tstop) <= troundoff) {
@@ -1477,7 +1477,7 @@ struct BDF {
       if (kflag != AMX_SUCCESS) return kflag;
       const double troundoff = FUZZ_FACTOR * UROUND * (fabs(tn) + fabs(h));
       if (fabs(tn - tstop) <= troundoff) tn = tstop;
-      if (!pb.on_step(tn, Z(0), Z(1), hscale)) return AMX_TRAJ_FULL;
+      if (!pb.on_step(tn, Z(0), Z(1), hscale, q)) return AMX_TRAJ_FULL;
       if (fabs(tn - tstop) <= troundoff) {
         if ((tout - tstop) * h >= 0.0 || fabs(tout - tstop) <= troundoff) {
           tstopset = false;

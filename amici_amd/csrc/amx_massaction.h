@@ -40,7 +40,8 @@ struct BatchArgs {
   // adjoint forward trajectory (Hermite data at every accepted step)
   double* traj_t;          // [nb][max_traj]
   double* traj_y;          // [nb][max_traj][NX]
-  double* traj_f;          // [nb][max_traj][NX]
+  double* traj_f;          // [nb][max_traj][NX] y' (Hermite interpolation) or null
+  int* traj_q;             // [nb][max_traj] order used (polynomial interpolation) or null
   int* traj_n;             // [nb]
   double* dJydx;           // [nb][nt][NX]
   int max_traj;
@@ -206,19 +207,28 @@ struct FwdProblem {
   double* tt;
   double* ty;
   double* tf;
+  int* tq;
   int ntraj, max_traj;
 
   __device__ __forceinline__ FwdProblem(const Team<T>& t, Smem<M>& sm, double* w)
-      : tm(t), s(sm), ev(t, sm), ws(w), tt(nullptr), ty(nullptr), tf(nullptr), ntraj(0),
-        max_traj(0) {}
+      : tm(t), s(sm), ev(t, sm), ws(w), tt(nullptr), ty(nullptr), tf(nullptr), tq(nullptr),
+        ntraj(0), max_traj(0) {}
 
-  __device__ __forceinline__ bool on_step(double tn, double z0, double z1, double hs) {
+  // the forward point after every accepted step (and the initial point):
+  // t, y and, for Hermite interpolation, y' = zn[1] / h
+  // (CVAhermiteStorePnt, cvodea.c:2839-2895), for polynomial interpolation the
+  // order used (CVApolynomialStorePnt, cvodea.c:3236-3262; 0 at the initial
+  // point)
+  __device__ __forceinline__ bool on_step(double tn, double z0, double z1, double hs, int qu) {
     if (tt == nullptr) return true;
     if (ntraj >= max_traj) return false;
-    if (tm.lane == 0) tt[ntraj] = tn;
+    if (tm.lane == 0) {
+      tt[ntraj] = tn;
+      if (tq) tq[ntraj] = qu;
+    }
     if (tm.lane < NX) {
       ty[(size_t)ntraj * NX + tm.lane] = z0;
-      tf[(size_t)ntraj * NX + tm.lane] = z1 / hs;
+      if (tf) tf[(size_t)ntraj * NX + tm.lane] = z1 / hs;
     }
     ntraj++;
     return true;
@@ -467,66 +477,138 @@ struct FwdProblem {
 };
 
 // ---------------------------------------------------------------------------
-// adjoint problem: xB' = -J(x(t))^T xB, qB' = -(df/dp)^T xB, x(t) by cubic
-// Hermite interpolation of the stored forward steps.
+// Forward solution x(t) during the adjoint sweep, from the forward steps
+// stored by the forward pass.  POLY (AMICI's default, CV_POLYNOMIAL): the
+// Newton form of the polynomial through order + 1 stored points, order being
+// the order used for the step that ends the interval, with divided
+// differences scaled by the interval length (CVApolynomialGetY,
+// cvodea.c:3270-3380); the interval search restates CVAfindIndex
+// (cvodea.c): t in (t[i-1], t[i]].  Otherwise cubic Hermite interpolation
+// between the interval's end points from the stored y and y'
+// (CVAhermiteGetY, cvodea.c:2906).
+template <class M, bool POLY>
+struct TrajInterp;
+
 template <class M>
-struct BwdProblem {
-  static constexpr int NX = M::NX, T = M::T, NQ = M::NP, NS = 0, NP = M::NP;
-  static constexpr int JLD = Smem<M>::JLD;
-  static constexpr int EPQ = (NQ + T - 1) / T;
-  const Team<T>& tm;
-  Smem<M>& s;
-  MAEval<M> ev;
+struct TrajInterp<M, true> {
+  static constexpr int NX = M::NX;
+  static constexpr int QM = LMAX;  // at most order 5: six points
   const double* tt;  // forward step times (increasing)
   const double* ty;  // [n][NX]
-  const double* tf;  // [n][NX]
-  int n;             // stored points
-  int cur;           // interval [cur-1, cur]
-  double ta, tb, ya, yb, fa, fb;
-  double t_cached, x_cached;
+  const int* tq;     // [n] order of the step ending at each stored time
+  int n, ilast, idx, order;
+  double dt, T[QM], Y[QM];  // points and divided differences (this lane's component)
 
-  __device__ __forceinline__ bool on_step(double, double, double, double) { return true; }
-
-  __device__ __forceinline__ BwdProblem(const Team<T>& t, Smem<M>& sm, const double* tt_,
-                                        const double* ty_, const double* tf_, int n_)
-      : tm(t), s(sm), ev(t, sm), tt(tt_), ty(ty_), tf(tf_), n(n_) {
-    cur = -1;
-    t_cached = -1.0;
-    x_cached = 0.0;
+  __device__ __forceinline__ void init(const double* tt_, const double* ty_, const int* tq_,
+                                       const double*, int n_) {
+    tt = tt_;
+    ty = ty_;
+    tq = tq_;
+    n = n_;
+    ilast = n - 1;
+    idx = -1;
+    order = 0;
+    dt = 1.0;
+#pragma unroll
+    for (int j = 0; j < QM; ++j) T[j] = Y[j] = 0.0;
   }
+  __device__ __forceinline__ double eval(double t, int lane) {
+    const int i = lane < NX ? lane : 0;
+    if (n < 2) return ty[i];
+    // CVAfindIndex: interval index with t in (tt[k-1], tt[k]]
+    int k = ilast;
+    bool newpoint = idx < 0;
+    if (t < tt[ilast - 1]) {
+      newpoint = true;
+      for (int g = 0; g < n && k > 0 && t <= tt[k - 1]; ++g) --k;
+      ilast = (k == 0) ? 1 : k;
+    } else if (t > tt[ilast]) {
+      newpoint = true;
+      for (int g = 0; g < n && k < n - 1 && t > tt[k]; ++g) ++k;
+      ilast = k;
+    }
+    if (k == 0) return ty[i];  // left limit
+    if (newpoint || k != idx) {
+      idx = k;
+      dt = fabs(tt[k] - tt[k - 1]);
+      int base = k;
+      order = tq[base];
+      if (k < order) base += order - k;
+#pragma unroll
+      for (int j = 0; j < QM; ++j) {
+        if (j <= order) {
+          T[j] = tt[base - j];
+          Y[j] = ty[(size_t)(base - j) * NX + i];
+        }
+      }
+#pragma unroll
+      for (int a = 1; a < QM; ++a) {
+#pragma unroll
+        for (int j = QM - 1; j >= a; --j) {
+          if (a <= order && j <= order) {
+            const double f = dt / (T[j] - T[j - a]);
+            Y[j] = f * (Y[j] - Y[j - 1]);  // N_VLinearSum(f, Y[j], -f, Y[j-1]): f (x - y)
+          }
+        }
+      }
+    }
+    // nested products c_0 = 1, c_{j+1} = c_j (t - T_j) / dt; x = sum_j c_j Y_j
+    double c = 1.0, x = Y[0];
+#pragma unroll
+    for (int j = 0; j < QM - 1; ++j) {
+      if (j < order) {
+        c = c * (t - T[j]) / dt;
+        x = fma(c, Y[j + 1], x);
+      }
+    }
+    return x;
+  }
+};
 
-  __device__ __forceinline__ void fetch(int k, double& t_, double& y_, double& f_) const {
-    const int i = tm.lane < NX ? tm.lane : 0;
+template <class M>
+struct TrajInterp<M, false> {
+  static constexpr int NX = M::NX;
+  const double* tt;  // forward step times (increasing)
+  const double* ty;  // [n][NX]
+  const double* tf;  // [n][NX] y'
+  int n, cur;        // interval [cur-1, cur]
+  double ta, tb, ya, yb, fa, fb;
+
+  __device__ __forceinline__ void init(const double* tt_, const double* ty_, const int*,
+                                       const double* tf_, int n_) {
+    tt = tt_;
+    ty = ty_;
+    tf = tf_;
+    n = n_;
+    cur = -1;
+    ta = tb = ya = yb = fa = fb = 0.0;
+  }
+  __device__ __forceinline__ void fetch(int k, int i, double& t_, double& y_, double& f_) const {
     t_ = tt[k];
     y_ = ty[(size_t)k * NX + i];
     f_ = tf[(size_t)k * NX + i];
   }
   // Selects the interval [tt[k-1], tt[k]]; a move to the previous interval
   // keeps the shared end point and loads only the new start point.
-  __device__ __forceinline__ void set_interval(int k) {
+  __device__ __forceinline__ void set_interval(int k, int i) {
     if (cur == k + 1) {
       tb = ta;
       yb = ya;
       fb = fa;
     } else {
-      fetch(k, tb, yb, fb);
+      fetch(k, i, tb, yb, fb);
     }
-    fetch(k - 1, ta, ya, fa);
+    fetch(k - 1, i, ta, ya, fa);
     cur = k;
   }
-
-  __device__ __forceinline__ double xf(double t) {
-    if (t == t_cached && cur > 0) return x_cached;
-    if (n < 2) {
-      x_cached = ty[tm.lane < NX ? tm.lane : 0];
-      t_cached = t;
-      return x_cached;
-    }
-    if (cur <= 0) set_interval(n - 1);
+  __device__ __forceinline__ double eval(double t, int lane) {
+    const int i = lane < NX ? lane : 0;
+    if (n < 2) return ty[i];
+    if (cur <= 0) set_interval(n - 1, i);
     // the backward sweep moves down through the stored steps; after a failed
     // step it may move up again (bounded walks: at most n intervals)
-    for (int g = 0; g < n && cur > 1 && t < ta; ++g) set_interval(cur - 1);
-    for (int g = 0; g < n && cur < n - 1 && t > tb; ++g) set_interval(cur + 1);
+    for (int g = 0; g < n && cur > 1 && t < ta; ++g) set_interval(cur - 1, i);
+    for (int g = 0; g < n && cur < n - 1 && t > tb; ++g) set_interval(cur + 1, i);
     const double hk = tb - ta;
     const double sx = (t - ta) / hk;
     const double om = 1.0 - sx;
@@ -534,8 +616,41 @@ struct BwdProblem {
     const double h10 = sx * om * om;
     const double h01 = sx * sx * (3.0 - 2.0 * sx);
     const double h11 = sx * sx * (sx - 1.0);
-    x_cached = h00 * ya + h10 * hk * fa + h01 * yb + h11 * hk * fb;
+    return h00 * ya + h10 * hk * fa + h01 * yb + h11 * hk * fb;
+  }
+};
+
+// adjoint problem: xB' = -J(x(t))^T xB, qB' = -(df/dp)^T xB, x(t) from the
+// stored forward steps (TrajInterp)
+template <class M, bool POLY>
+struct BwdProblem {
+  static constexpr int NX = M::NX, T = M::T, NQ = M::NP, NS = 0, NP = M::NP;
+  static constexpr int JLD = Smem<M>::JLD;
+  static constexpr int EPQ = (NQ + T - 1) / T;
+  const Team<T>& tm;
+  Smem<M>& s;
+  MAEval<M> ev;
+  TrajInterp<M, POLY> ip;
+  double t_cached, x_cached;
+  bool have_cached;
+
+  __device__ __forceinline__ bool on_step(double, double, double, double, int) { return true; }
+
+  __device__ __forceinline__ BwdProblem(const Team<T>& t, Smem<M>& sm, const double* tt_,
+                                        const double* ty_, const int* tq_, const double* tf_,
+                                        int n_)
+      : tm(t), s(sm), ev(t, sm) {
+    ip.init(tt_, ty_, tq_, tf_, n_);
+    t_cached = 0.0;
+    x_cached = 0.0;
+    have_cached = false;
+  }
+
+  __device__ __forceinline__ double xf(double t) {
+    if (have_cached && t == t_cached) return x_cached;
+    x_cached = ip.eval(t, tm.lane);
     t_cached = t;
+    have_cached = true;
     return x_cached;
   }
 
@@ -839,7 +954,8 @@ __device__ __forceinline__ void simulate_forward(int inst, const BatchArgs& a, c
   if constexpr (MODE == 2) {
     pb0.tt = a.traj_t + I * (size_t)a.max_traj;
     pb0.ty = a.traj_y + I * (size_t)a.max_traj * NX;
-    pb0.tf = a.traj_f + I * (size_t)a.max_traj * NX;
+    pb0.tf = a.traj_f ? a.traj_f + I * (size_t)a.max_traj * NX : nullptr;
+    pb0.tq = a.traj_q ? a.traj_q + I * (size_t)a.max_traj : nullptr;
     pb0.max_traj = a.max_traj;
   }
   pb0.mat = mat;
@@ -958,11 +1074,11 @@ __device__ __forceinline__ void simulate_forward(int inst, const BatchArgs& a, c
 // backward (adjoint) sweep of one instance; EventHandlingBwdSimulator::run
 // (src/backwardproblem.cpp:267-332) with a fresh order-1 restart after every
 // data-point jump (CVodeSolver::reinit_b, src/solver_cvodes.cpp:594-649).
-template <class M>
+template <class M, bool POLY>
 __device__ __forceinline__ void simulate_backward(int inst, const BatchArgs& a, const Team<M::T>& tm,
                                                   Smem<M>& s, BdfLds<M::NX, M::NP>& lds) {
   constexpr int NX = M::NX, NP = M::NP, T = M::T;
-  using PB = BwdProblem<M>;
+  using PB = BwdProblem<M, POLY>;
   constexpr int EPQ = PB::EPQ;
   const size_t I = (size_t)inst;
   int* const diag = a.diag ? a.diag + I * a.nt * AMX_NDIAG : nullptr;
@@ -979,7 +1095,8 @@ __device__ __forceinline__ void simulate_backward(int inst, const BatchArgs& a, 
   const int n = a.traj_n[inst];
   const double* tt = a.traj_t + I * (size_t)a.max_traj;
   PB pb0(tm, s, tt, a.traj_y + I * (size_t)a.max_traj * NX,
-         a.traj_f + I * (size_t)a.max_traj * NX, n);
+         POLY ? a.traj_q + I * (size_t)a.max_traj : nullptr,
+         POLY ? nullptr : a.traj_f + I * (size_t)a.max_traj * NX, n);
   constexpr bool BIG = big_model<M>();
   if constexpr (BIG) pb0.mat = a.lu_ws + I * (size_t)NX * NX;
   BDF<PB> bdf(pb0, lds, a.spB);  // holds pb0 and the LDS block by reference
@@ -1145,7 +1262,7 @@ __device__ __forceinline__ void forward_kernel_body(BatchArgs a) {
   }
 }
 
-template <class M>
+template <class M, bool POLY>
 __device__ __forceinline__ void backward_kernel_body(BatchArgs a) {
   constexpr int T = M::T;
   __shared__ Smem<M> smem[Launch<M>::TEAMS];
@@ -1156,7 +1273,7 @@ __device__ __forceinline__ void backward_kernel_body(BatchArgs a) {
   for (int guard = 0; guard <= a.nb; ++guard) {
     const int inst = next_instance<M>(tm, a.counter);
     if (inst >= a.nb) break;
-    simulate_backward<M>(inst, a, tm, s, lds);
+    simulate_backward<M, POLY>(inst, a, tm, s, lds);
   }
 }
 

@@ -211,6 +211,7 @@ class Solver:
     ss_atol: float = math.nan
     rdata_reporting: RDataReporting = RDataReporting.full
     max_blocks: int = 0  # launch shape: 0 = as many blocks as fit the device at once
+    interp: int = 2      # InterpolationType: 1 hermite, 2 polynomial (AMICI's default)
 
     def set_relative_tolerance(self, v): self.rtol = float(v)
     def get_relative_tolerance(self): return self.rtol
@@ -238,6 +239,21 @@ class Solver:
     def set_sensitivity_method(self, v): self.sensi_meth = SensitivityMethod(int(v))
     def get_sensitivity_method(self): return self.sensi_meth
 
+    def set_interpolation_type(self, v):
+        """Interpolation of the forward solution in the adjoint sweep
+        (Solver::setInterpolationType; InterpolationType hermite = 1,
+        polynomial = 2, the default)."""
+        name = getattr(v, "name", str(v)).lower()
+        if name in ("hermite", "1"):
+            self.interp = 1
+        elif name in ("polynomial", "2"):
+            self.interp = 2
+        else:
+            raise ValueError(f"unknown interpolation type {v!r}")
+
+    def get_interpolation_type(self):
+        return self.interp
+
     def set_max_blocks(self, v):
         """Caps the blocks of the persistent kernels (no counterpart in the
         reference): with fewer blocks than instances each block takes further
@@ -263,7 +279,7 @@ class Solver:
                               maxstepsB=self.maxstepsB, max_traj=self.max_traj,
                               maxtime=self.maxtime, newton_maxsteps=self.newton_maxsteps,
                               ss_rtol=self.ss_rtol, ss_atol=self.ss_atol,
-                              max_blocks=self.max_blocks)
+                              max_blocks=self.max_blocks, interp=self.interp)
 
 
 class ExpData:

@@ -26,11 +26,16 @@
 extern "C" {
 #endif
 
-#define AMX_ABI_VERSION 3
+#define AMX_ABI_VERSION 4
 #define AMX_NSTATS 16
 #define AMX_NDIAG 9
 /* bytes of the static workspace of amx_run_batch_static, per GPU */
 #define AMX_STATIC_WORKSPACE_BYTES (64u << 20)
+
+/* interpolation of the forward solution in the adjoint sweep
+ * (amici::InterpolationType, include/amici/defines.h; CV_HERMITE / CV_POLYNOMIAL) */
+#define AMX_INTERP_HERMITE 1
+#define AMX_INTERP_POLYNOMIAL 2
 
 /* sensitivity method (amici::SensitivityMethod, include/amici/defines.h) */
 #define AMX_SENSI_NONE 0
@@ -56,6 +61,8 @@ typedef struct {
                                   instances run at a time and each block takes further
                                   instances from the work queue (no counterpart in the
                                   reference: a launch-shape control)                    */
+    int interp;                /* adjoint: AMX_INTERP_POLYNOMIAL (AMICI's default; 0 means
+                                  the same) or AMX_INTERP_HERMITE (Solver::setInterpolationType) */
 } amx_solver_opts;
 
 typedef struct {

@@ -50,8 +50,9 @@ def test_workspace_bytes(lib):
     o = _lib.make_opts(sensi_meth=_lib.SENSI_ADJOINT, maxsteps=1000)
     b1 = L.workspace_bytes(o, 1, 10)
     b8 = L.workspace_bytes(o, 8, 10)
-    # trajectory: t, y, f per stored step and instance dominate
-    assert b8 - b1 >= 7 * 1001 * (1 + 2 * net.nx) * 8
+    # trajectory: t, y and the order per stored step and instance dominate
+    # (polynomial interpolation, the default)
+    assert b8 - b1 >= 7 * 1001 * ((1 + net.nx) * 8 + 4)
     o.max_traj = 100
     assert L.workspace_bytes(o, 8, 10) < b8
     f = _lib.make_opts(sensi_meth=_lib.SENSI_FORWARD)
@@ -162,14 +163,31 @@ def test_teams_narrower_than_a_wavefront_are_refused():
         codegen.emit_model_traits(networks.network_n40(), 32)
 
 
-def test_abi_v3_struct_layout():
-    """amx_solver_opts ends with max_blocks and amx_batch_out with diag
-    (include/amx.h, ABI version 3)."""
-    assert _lib.AMX_ABI_VERSION == 3
-    assert _lib.SolverOpts._fields_[-1][0] == "max_blocks"
+def test_abi_v4_struct_layout():
+    """amx_solver_opts ends with max_blocks and interp, amx_batch_out with diag
+    (include/amx.h, ABI version 4)."""
+    assert _lib.AMX_ABI_VERSION == 4
+    assert [f[0] for f in _lib.SolverOpts._fields_[-2:]] == ["max_blocks", "interp"]
     assert _lib.BatchOut._fields_[-1][0] == "diag"
     text = (ROOT / "include" / "amx.h").read_text()
-    assert "#define AMX_ABI_VERSION 3" in text and "#define AMX_NDIAG 9" in text
+    assert "#define AMX_ABI_VERSION 4" in text and "#define AMX_NDIAG 9" in text
+    assert "#define AMX_INTERP_HERMITE 1" in text and "#define AMX_INTERP_POLYNOMIAL 2" in text
+
+
+def test_interpolation_workspace(lib):
+    """Polynomial interpolation stores the order per step instead of y'."""
+    net, L = lib
+    poly = _lib.make_opts(sensi_meth=_lib.SENSI_ADJOINT, maxsteps=1000)
+    herm = _lib.make_opts(sensi_meth=_lib.SENSI_ADJOINT, maxsteps=1000, interp=_lib.INTERP_HERMITE)
+    d = L.workspace_bytes(herm, 8, 10) - L.workspace_bytes(poly, 8, 10)
+    assert abs(d - 8 * 1001 * (8 * net.nx - 4)) <= 512
+    bad = _lib.make_opts(sensi_meth=_lib.SENSI_ADJOINT, interp=3)
+    buf = (ctypes.c_double * 4)()
+    p = ctypes.cast(buf, ctypes.c_void_p).value
+    ins = _lib.BatchIn(1, 3, 0, p, p if net.nk else None, p, None, p, None)
+    outs = _lib.BatchOut(None, None, None, p, p, p, None, None)
+    assert L.lib.amx_run_batch(ctypes.byref(bad), ctypes.byref(ins), ctypes.byref(outs), p, 8,
+                               None) == -22
 
 
 def test_negative_max_blocks_rejected(lib):

@@ -52,17 +52,23 @@ def _entrywise(a, ref, floor):
 
 
 def _gate(gpu_err, cv_err, what, cap=None):
-    """Per instance and for the worst / median instance: GPU error within
-    FACTOR x CVODES's own error at the same tolerances (per instance, the
-    larger of that instance's and the median CVODES error)."""
-    med = np.median(cv_err)
-    bound = FACTOR * np.maximum(cv_err, med)
+    """The batch's error distribution against CVODES's at the same
+    tolerances: the median, the 90th percentile and the largest GPU error
+    within FACTOR x CVODES's, and each instance within FACTOR x the larger of
+    CVODES's error for it and CVODES's 90th percentile.  (At rtol 1e-8 the
+    error of one condition is itself a chaotic function of round-off: for a
+    Robertson condition CVODES's own normwise gradient error ranges from
+    2e-6 to 1.2e-4 under 1e-10 perturbations of log10 k, so a single CVODES
+    run does not bound one instance's error.)"""
+    q90 = np.quantile(cv_err, 0.9)
+    bound = FACTOR * np.maximum(cv_err, q90)
     worst = int(np.argmax(gpu_err / bound))
     assert (gpu_err <= bound).all(), (
         f"{what}: instance {worst} error {gpu_err[worst]:.3g} > {FACTOR} x CVODES "
-        f"{max(cv_err[worst], med):.3g}")
+        f"{max(cv_err[worst], q90):.3g}")
     assert gpu_err.max() <= FACTOR * cv_err.max(), (what, gpu_err.max(), cv_err.max())
-    assert np.median(gpu_err) <= FACTOR * med, (what, np.median(gpu_err), med)
+    assert np.quantile(gpu_err, 0.9) <= FACTOR * q90, (what, np.quantile(gpu_err, 0.9), q90)
+    assert np.median(gpu_err) <= FACTOR * np.median(cv_err), (what, np.median(gpu_err))
     if cap is not None:
         assert gpu_err.max() <= cap, (what, gpu_err.max(), cap)
 

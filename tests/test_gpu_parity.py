@@ -253,3 +253,18 @@ def test_n200_preequilibration(gpu, meth):
     _check_llh(model.network, out, g)
     if meth:
         _close(out["sllh"], g["sllh_asa"], 1e-5, 1e-7, "sllh (adjoint)")
+
+
+@pytest.mark.parametrize("model,fixture", [("robertson", "robertson_batch64"),
+                                           ("n40", "n40_conditions16")])
+def test_adjoint_gradient_hermite(models, model, fixture):
+    """The adjoint sweep with Hermite interpolation of the forward solution
+    (InterpolationType hermite) against the oracle's adjoint gradient (AMICI's
+    default polynomial interpolation): both interpolants are accurate to the
+    integration tolerance, so the gradients agree to the parity bar."""
+    from amici_amd import sim
+    g = _load(fixture)
+    out = _run(models, model, g, sim.SensitivityMethod.adjoint, interp=1)
+    assert (out["status"] == 0).all(), out["status"]
+    _check_llh(models[model].network, out, g)
+    _close(out["sllh"], g["sllh_asa"], 1e-5, 1e-7, "sllh (adjoint, Hermite)")

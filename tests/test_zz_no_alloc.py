@@ -80,7 +80,8 @@ def test_static_workspace_limit_refused(gpu):
     from amici_amd import sim
     g = dict(np.load(GOLD / "n40_conditions16.npz"))
     model = sim.Model(networks.network_n40())
-    sol = sim.Solver(**BENCH)  # max_traj = maxsteps + 1: 6.5 MB per instance
+    # max_traj = maxsteps + 1: 30001 stored steps of (t, x, order), 10 MB per instance
+    sol = sim.Solver(**{**BENCH, "maxsteps": 30000})
     sol.set_sensitivity_order(sim.SensitivityOrder.first)
     sol.set_sensitivity_method(sim.SensitivityMethod.adjoint)
     p = _structs(model, sol, g, 16, gpu)
