@@ -101,27 +101,35 @@ struct BdfLds {
   double tau[LMAX + 1];   // previous step sizes
   double l[LMAX];         // method coefficients
   double tq[6];           // error-test constants
-  int perm[NX];           // perm[k] = lane of the k-th pivot row
+  unsigned int permw[(NLU + 3) / 4];  // lane of the k-th pivot row: byte k % 4 of word k / 4
+  double wsol[NLU];       // solve: y_k, then x_k, of pivot step k
 };
 
 // Row-per-lane dense LU of an NX x NX matrix held by the first NX team lanes.
-// The factors are kept in LDS (BdfLds::lu, perm); the per-lane pivot
-// bookkeeping lives in registers.
+// The factors are kept in LDS (BdfLds::lu, permw); the per-lane pivot
+// bookkeeping lives in registers.  Neither the factorisation nor the solve
+// compares the lane index with a loop constant (such per-step lane masks are
+// loop invariant, and the compiler keeps them in scalar registers for the
+// whole kernel): the pivot order is packed four steps to a 32-bit word, the
+// pivot search is a maximum reduction and a wave ballot, and the
+// substitutions move values between lanes with readlane / writelane only.
 template <int NX, int T>
 struct LURows {
+  static constexpr int NW = (NX + 3) / 4;
   double dinv;  // 1 / pivot of the row this lane holds
   int pos;      // elimination step at which this lane's row was the pivot
-  int src;      // lane that holds pivot row number `lane`
 
-  // Factorises the rows a[] (lane i: row i) and stores the factors in S.
-  // Returns false for a zero or non-finite pivot (recoverable setup failure).
+  // Factorises the rows a[] (lane i: row i) and stores the factors in S
+  // (row i, column j at lu[j][i]: the multipliers left of the row's pivot
+  // step, the row of U from it on).  Returns false for a zero or non-finite
+  // pivot (recoverable setup failure).
   template <class L>
   __device__ __forceinline__ bool factor(const Team<T>& tm, double (&a)[NX], L& S) {
     const bool act = tm.lane < NX;
     pos = act ? -1 : NX;
-    src = 0;
     dinv = 0.0;
     bool ok = true;
+    unsigned int pw = 0;  // pivots of the current group of four steps
 #pragma unroll
     for (int k = 0; k < NX; ++k) {
       if (!ok) continue;  // (team-uniform; keeps the loop fully unrolled)
@@ -133,18 +141,20 @@ struct LURows {
         ok = false;
         continue;
       }
-      S.perm[k] = p;
-      if (tm.lane == k) src = p;
-      double prow[NX];
-#pragma unroll
-      for (int j = k; j < NX; ++j) prow[j] = tm.bcast(a[j], p);
-      const double inv = 1.0 / prow[k];
+      pw |= (unsigned int)p << (8 * (k & 3));
+      if ((k & 3) == 3 || k == NX - 1) {
+        if (tm.lane == 0) S.permw[k >> 2] = pw;
+        pw = 0;
+      }
+      const double inv = 1.0 / tm.bcast(a[k], p);
       if (tm.lane == p) { pos = k; dinv = inv; }
-      if (pos < 0) {
-        const double l = a[k] * inv;
-        a[k] = l;
+      const bool upd = pos < 0;
+      const double l = a[k] * inv;
+      if (upd) a[k] = l;
 #pragma unroll
-        for (int j = k + 1; j < NX; ++j) a[j] = fma(-prow[j], l, a[j]);
+      for (int j = k + 1; j < NX; ++j) {
+        const double pj = tm.bcast(a[j], p);
+        if (upd) a[j] = fma(-pj, l, a[j]);
       }
     }
     if (act) {
@@ -154,32 +164,46 @@ struct LURows {
     return ok;
   }
 
-  // Solves (LU) x = b; b is this lane's component of the right-hand side.
-  // The lane's row of factors and the pivot order are loaded from LDS once,
-  // before the substitution loops, so the loops run on registers.
+  // Solves (LU) x = b; b is this lane's component of the right-hand side
+  // (row order) and the result is x_lane.  Forward substitution: step k reads
+  // y_k = b from the k-th pivot lane, and every lane subtracts its entry of
+  // column k times y_k; lanes whose row was pivoted earlier collect
+  // meaningless updates, so y_k is also kept in LDS (wsol[k]), from where
+  // every lane restarts the back substitution with y of its own pivot step.
+  // Back substitution likewise: x_k = b / pivot on lane p_k, kept in wsol[k],
+  // and x_lane is read from there at the end.
   template <class L>
-  __device__ __forceinline__ double solve(double b, const Team<T>& tm, const L& S) const {
+  __device__ __forceinline__ double solve(double b, const Team<T>& tm, L& S) const {
     const bool act = tm.lane < NX;
     const int cl = act ? tm.lane : NX - 1;
+    unsigned int pw[NW];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) pw[w] = __builtin_amdgcn_readfirstlane(S.permw[w]);
     double r[NX];
-    int pv[NX];
+#pragma unroll
+    for (int k = 0; k < NX; ++k) r[k] = S.lu[k][cl];
+    const bool writer = tm.lane == 0;
 #pragma unroll
     for (int k = 0; k < NX; ++k) {
-      r[k] = S.lu[k][cl];
-      pv[k] = __builtin_amdgcn_readfirstlane(S.perm[k]);
+      const int pk = (pw[k >> 2] >> (8 * (k & 3))) & 0xFF;
+      const double yk = tm.bcast(b, pk);
+      if (writer) S.wsol[k] = yk;
+      b = fma(-r[k], yk, b);
     }
-#pragma unroll
-    for (int k = 0; k < NX; ++k) {
-      const double yk = tm.bcast(b, pv[k]);
-      if (act && pos > k) b = fma(-r[k], yk, b);
-    }
+    team_sync();
+    b = S.wsol[act ? pos : 0];
+    team_sync();
 #pragma unroll
     for (int k = NX - 1; k >= 0; --k) {
-      const double xk = tm.bcast(b * dinv, pv[k]);
-      if (pos == k) b = xk;
-      else if (act && pos < k) b = fma(-r[k], xk, b);
+      const int pk = (pw[k >> 2] >> (8 * (k & 3))) & 0xFF;
+      const double xk = tm.bcast(b * dinv, pk);
+      if (writer) S.wsol[k] = xk;
+      b = fma(-r[k], xk, b);
     }
-    return tm.gather(b, src);
+    team_sync();
+    const double x = S.wsol[cl];
+    team_sync();
+    return x;
   }
 };
 
@@ -583,7 +607,7 @@ struct BDF {
   }
   // true when any lane holds a non-finite value (NaN-safe, team-uniform)
   __device__ __forceinline__ bool any_nonfinite(double v) const {
-    return tm.maxi((active && !isfinite(v)) ? 1 : 0) != 0;
+    return tm.any(active && !isfinite(v));
   }
   __device__ __forceinline__ void set_ewt() {
     ewt = active ? 1.0 / (sp.rtol * fabs(Z(0)) + sp.atol) : 0.0;
@@ -1457,18 +1481,29 @@ struct BDF {
       }
       if (nstloc >= mxstep) return AMX_TOO_MUCH_WORK;
       if (sp.deadline != 0 && wall_clock64() > sp.deadline) return AMX_MAX_TIME_EXCEEDED;
-      double nrm = wrms(Z(0), ewt);
-      if constexpr (QUAD) {
-        if (quad_errcon()) {
-          double z0[EPQ];
-          znQ0(z0);
-          nrm = fmax(nrm, wrmsQ(z0, ewtQ));
+      // CVode's "too much accuracy" test, tolsf = UROUND * WRMS(zn[0]) > 1.  With
+      // the weights ewt = 1 / (rtol |zn0| + atol) just computed from zn[0]
+      // (atol > 0) every weighted component is below 1 / rtol, so the test
+      // can only fail when a relative tolerance is below UROUND; the norms are
+      // computed only then, and otherwise a non-finite state is caught by a
+      // ballot.
+      if (sp.rtol < UROUND || (QUAD && quad_errcon() && sp.rtolQ < UROUND) ||
+          (SENS && sp.rtolS < UROUND)) {
+        double nrm = wrms(Z(0), ewt);
+        if constexpr (QUAD) {
+          if (quad_errcon()) {
+            double z0[EPQ];
+            znQ0(z0);
+            nrm = fmax(nrm, wrmsQ(z0, ewtQ));
+          }
         }
+        if constexpr (SENS) nrm = fmax(nrm, pb.sens_norm_zn(0));
+        if (!(nrm == nrm)) return AMX_ERR_FAILURE;
+        tolsf = UROUND * nrm;
+        if (tolsf > 1.0) return AMX_TOO_MUCH_ACC;
+      } else if (any_nonfinite(Z(0))) {
+        return AMX_ERR_FAILURE;
       }
-      if constexpr (SENS) nrm = fmax(nrm, pb.sens_norm_zn(0));
-      if (!(nrm == nrm)) return AMX_ERR_FAILURE;
-      tolsf = UROUND * nrm;
-      if (tolsf > 1.0) return AMX_TOO_MUCH_ACC;
       tolsf = 1.0;
       AMX_PF_END(tl, PF_LOOP);
       AMX_PF_BEGIN(ts);

@@ -118,6 +118,18 @@ struct Wave64 {
     u = __builtin_amdgcn_readlane(v, 48); r = u > r ? u : r;
     return r;
   }
+  // Maximum and the lowest wave lane that holds it (a maximum reduction and
+  // a ballot of the lanes equal to it); v must not be NaN.
+  static __device__ __forceinline__ double argmax_lane(double v, int& idx) {
+    const double m = max(v);
+    const unsigned long long bits = __builtin_amdgcn_ballot_w64(v == m);
+    idx = (int)__builtin_ctzll(bits | (1ull << 63));
+    return m;
+  }
+  // true when the predicate holds on any lane of the wave (uniform)
+  static __device__ __forceinline__ bool any(bool pred) {
+    return __builtin_amdgcn_ballot_w64(pred) != 0;
+  }
   // (value, index) maximum; ties resolve to the smaller index.
   static __device__ __forceinline__ double argmax(double v, int i, int& idx) {
 #define AMX_ARGMAX_STEP(CTRL)                                \
@@ -196,6 +208,11 @@ struct Team {
       return Wave64::max(v);
     }
   }
+  // true when pred holds on any lane of the team (team-uniform)
+  __device__ __forceinline__ bool any(bool pred) const {
+    if constexpr (T > 64) return maxi(pred ? 1 : 0) != 0;
+    else return Wave64::any(pred);
+  }
   __device__ __forceinline__ int maxi(int v) const {
     if constexpr (T > 64) {
       const int wv = Wave64::maxi(v);
@@ -234,7 +251,7 @@ struct Team {
       idx = bi;
       return bv;
     } else {
-      return Wave64::argmax(v, lane, idx);
+      return Wave64::argmax_lane(v, idx);
     }
   }
   // Value of v on team lane src (src uniform within the team).
