@@ -9,11 +9,12 @@ import ctypes
 import math
 from pathlib import Path
 
-AMX_ABI_VERSION = 4
-AMX_NSTATS = 16
+AMX_ABI_VERSION = 5
+AMX_NSTATS = 18
 AMX_NDIAG = 9
 SENSI_NONE, SENSI_FORWARD, SENSI_ADJOINT = 0, 1, 2
 INTERP_HERMITE, INTERP_POLYNOMIAL = 1, 2
+STLDET_ON, STLDET_OFF = 1, 2
 
 SYMBOLS = ("amx_model_info", "amx_workspace_bytes", "amx_run_batch", "amx_run_sequential",
            "amx_set_trace", "amx_set_timing", "amx_kernel_times", "amx_status_string",
@@ -33,6 +34,7 @@ class SolverOpts(ctypes.Structure):
         ("newton_maxsteps", ctypes.c_int),
         ("ss_rtol", ctypes.c_double), ("ss_atol", ctypes.c_double),
         ("max_blocks", ctypes.c_int), ("interp", ctypes.c_int),
+        ("stldet", ctypes.c_int),
     ]
 
 
@@ -131,8 +133,8 @@ def make_opts(rtol=1e-8, atol=1e-16, sensi_meth=SENSI_NONE, maxsteps=10000, rtol
               atol_fsa=math.nan, rtolB=math.nan, atolB=math.nan, quad_rtol=1e-8,
               quad_atol=1e-12, maxstepsB=0, max_traj=0, maxtime=0.0, newton_maxsteps=40,
               ss_rtol=math.nan, ss_atol=math.nan, max_blocks=0,
-              interp=INTERP_POLYNOMIAL) -> SolverOpts:
+              interp=INTERP_POLYNOMIAL, stldet=True) -> SolverOpts:
     return SolverOpts(rtol, atol, rtol_fsa, atol_fsa, rtolB, atolB, quad_rtol, quad_atol,
                       int(maxsteps), int(maxstepsB), int(sensi_meth), int(max_traj),
                       float(maxtime), int(newton_maxsteps), float(ss_rtol), float(ss_atol),
-                      int(max_blocks), int(interp))
+                      int(max_blocks), int(interp), STLDET_ON if stldet else STLDET_OFF)

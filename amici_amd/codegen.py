@@ -139,6 +139,53 @@ _INT_TABLES = ["RA", "RB", "KI", "SP_PTR", "SP_R", "RS_PTR", "RS_S", "J_I", "J_J
 _DBL_TABLES = ["SP_C", "RS_C", "JT_C", "DX_C", "OB_C", "OT_C", "X0C"]
 
 
+def _lane_major(rows, team: int, pad):
+    """Pads the entry lists ``rows`` (row l is handled by team lane l % team in
+    pass l // team) to the longest row and lays them out lane-major: entry q
+    of lane i in pass e at [(e * n + q) * team + i], so that one fixed-length,
+    fully unrolled loop over q reads each entry of all lanes with one
+    coalesced load.  Returns n and the flat list."""
+    passes = max(1, -(-len(rows) // team))
+    n = max([len(r) for r in rows] + [1])
+    flat = [pad] * (passes * n * team)
+    for l, row in enumerate(rows):
+        e, i = divmod(l, team)
+        for q, v in enumerate(row):
+            flat[(e * n + q) * team + i] = v
+    return n, flat
+
+
+def lane_tables(net: ReactionNetwork, team: int, t: dict | None = None) -> dict:
+    """Lane-major padded forms of the per-lane sparse sums (model_tables):
+    DXL / DXLC (J^T v, lane = species: entry r | o' << 16 with o' = NX for a
+    unimolecular entry, whose factor x[NX] is 1, and the coefficient), SPL /
+    SPLC (S w, lane = species), RSL / RSLC (S^T v, lane = reaction) and PRL /
+    PRLC (df/dp^T, lane = parameter).  Padding entries have coefficient 0 and
+    indices inside the arrays they address."""
+    t = t or model_tables(net)
+    nx = net.nx
+
+    def rows(ptr, *cols):
+        return [list(zip(*(c[ptr[i]:ptr[i + 1]] for c in cols))) for i in range(len(ptr) - 1)]
+
+    out = {}
+    dx = rows(t["DX_PTR"], t["DX_R"], t["DX_O"], t["DX_C"])
+    n, idx = _lane_major([[r | ((o if o >= 0 else nx) << 16) for r, o, _ in row] for row in dx],
+                         team, nx << 16)
+    out["DXL_N"], out["DXL"] = n, idx
+    out["DXLC"] = _lane_major([[c for _, _, c in row] for row in dx], team, 0.0)[1]
+    sp = rows(t["SP_PTR"], t["SP_R"], t["SP_C"])
+    out["SPL_N"], out["SPL"] = _lane_major([[r for r, _ in row] for row in sp], team, 0)
+    out["SPLC"] = _lane_major([[c for _, c in row] for row in sp], team, 0.0)[1]
+    rs = rows(t["RS_PTR"], t["RS_S"], t["RS_C"])
+    out["RSL_N"], out["RSL"] = _lane_major([[i for i, _ in row] for row in rs], team, 0)
+    out["RSLC"] = _lane_major([[c for _, c in row] for row in rs], team, 0.0)[1]
+    pr = rows(t["PR_PTR"], t["PR_R"])
+    out["PRL_N"], out["PRL"] = _lane_major([[r for (r,) in row] for row in pr], team, 0)
+    out["PRLC"] = _lane_major([[1.0 for _ in row] for row in pr], team, 0.0)[1]
+    return out
+
+
 def _fmt_array(ctype: str, name: str, vals) -> str:
     vals = list(vals) if len(vals) else [0]
     if ctype == "double":
@@ -177,6 +224,12 @@ def emit_model_traits(net: ReactionNetwork, team: int | None = None) -> str:
         lines.append(_fmt_array("int", name, t[name]))
     for name in _DBL_TABLES:
         lines.append(_fmt_array("double", name, t[name]))
+    lt = lane_tables(net, team, t)
+    for name, ctype in (("DXL", "unsigned"), ("DXLC", "double"), ("SPL", "int"), ("SPLC", "double"),
+                        ("RSL", "int"), ("RSLC", "double"), ("PRL", "int"), ("PRLC", "double")):
+        if name + "_N" in lt:
+            lines.append(f"  static constexpr int {name}_N = {lt[name + '_N']};\n")
+        lines.append(_fmt_array(ctype, name, lt[name]))
     lines.append("};\n\n")
     return "".join(lines)
 

@@ -11,6 +11,7 @@
 //   cvDoErrorTest                :7461           cvCompleteStep  :7634
 //   cvPrepareNextStep/SetEta     :7747-7830      cvComputeEtaqm1/qp1, cvChooseEta :7836-8010
 //   cvHin / cvUpperBoundH0 / cvYddNorm :5436-5798 CVode tstop handling :3100-3620
+//   stability-limit detection    cvBDFStab :8158-8219, cvSLdet :8256 (amx_stald.h)
 //   constants                    cvodes_impl.h:57-204, cvodes_ls_impl.h:40-42
 //   quadratures (cvQuadNls)      cvodes.c:7089;  staggered FSA (cvStgrNls) :7167,
 //                                cvodes_nls_stg.c (residual / lsolve / conv test)
@@ -21,10 +22,11 @@
 // Storage: the Nordsieck history, the LU factors and the method coefficient
 // arrays of an instance live in LDS (BdfLds); scalars and per-lane work values
 // live in registers.  Every function is inlined into the batch kernels.
-// Not modelled: BDF stability-limit detection (cvBDFStab), hmin/hmax limits.
+// Not modelled: hmin/hmax limits (AMICI sets neither).
 #pragma once
 #include <type_traits>
 
+#include "amx_stald.h"
 #include "amx_team.h"
 
 namespace amx {
@@ -103,6 +105,8 @@ struct BdfLds {
   double tq[6];           // error-test constants
   unsigned int permw[(NLU + 3) / 4];  // lane of the k-th pivot row: byte k % 4 of word k / 4
   double wsol[NLU];       // solve: y_k, then x_k, of pivot step k
+  double ssdat[5][3];     // stability-limit detection: squared scaled derivative norms
+                          // of orders q-1, q, q+1 of the last five steps, newest first
 };
 
 // Row-per-lane dense LU of an NX x NX matrix held by the first NX team lanes.
@@ -168,7 +172,8 @@ struct LURows {
   // (row order) and the result is x_lane.  Forward substitution: step k reads
   // y_k = b from the k-th pivot lane, and every lane subtracts its entry of
   // column k times y_k; lanes whose row was pivoted earlier collect
-  // meaningless updates, so y_k is also kept in LDS (wsol[k]), from where
+  // meaningless updates, so y_k is also kept in LDS (wsol[k], stored by every
+  // lane with the same value, which costs less than a lane test), from where
   // every lane restarts the back substitution with y of its own pivot step.
   // Back substitution likewise: x_k = b / pivot on lane p_k, kept in wsol[k],
   // and x_lane is read from there at the end.
@@ -182,12 +187,11 @@ struct LURows {
     double r[NX];
 #pragma unroll
     for (int k = 0; k < NX; ++k) r[k] = S.lu[k][cl];
-    const bool writer = tm.lane == 0;
 #pragma unroll
     for (int k = 0; k < NX; ++k) {
       const int pk = (pw[k >> 2] >> (8 * (k & 3))) & 0xFF;
       const double yk = tm.bcast(b, pk);
-      if (writer) S.wsol[k] = yk;
+      S.wsol[k] = yk;
       b = fma(-r[k], yk, b);
     }
     team_sync();
@@ -197,7 +201,7 @@ struct LURows {
     for (int k = NX - 1; k >= 0; --k) {
       const int pk = (pw[k >> 2] >> (8 * (k & 3))) & 0xFF;
       const double xk = tm.bcast(b * dinv, pk);
-      if (writer) S.wsol[k] = xk;
+      S.wsol[k] = xk;
       b = fma(-r[k], xk, b);
     }
     team_sync();
@@ -497,6 +501,7 @@ struct SolverParams {
   double rtolS, atolS;
   int maxsteps;
   int errconQ;
+  int stldet;                   // BDF stability-limit detection on (CVodeSetStabLimDet)
   unsigned long long deadline;  // wall_clock64() limit, 0 = none
 };
 
@@ -542,7 +547,7 @@ struct BDF {
   // values returned by advance(): this lane's state and quadratures
   double yout, yQout[EPQ];
   // counters
-  int nfe, nsetups, nje, nni, netf, ncfn, nfQe, nniS;
+  int nfe, nsetups, nje, nni, netf, ncfn, nfQe, nniS, nor;
   // optional step trace (team lane 0 writes records of TRACE_REC doubles)
   double* trace;
   int trace_cap, trace_n;
@@ -641,6 +646,8 @@ struct BDF {
       for (int j = 0; j <= LMAX; ++j) S.tau[j] = 0.0;
       for (int j = 0; j < LMAX; ++j) S.l[j] = 0.0;
       for (int j = 0; j < 6; ++j) S.tq[j] = 0.0;
+      for (int i = 0; i < 5; ++i)
+        for (int k = 0; k < 3; ++k) S.ssdat[i][k] = 0.0;
     }
     team_sync();
     q = 1; L_ = 2; qwait = L_; qprime = 1; etamax = ETA_MAX_FS;
@@ -651,7 +658,7 @@ struct BDF {
     jcur = false; tstopset = false; at_tstop = false; tstop = 0.0;
     etaq = etaqm1 = etaqp1 = 0.0;
     yout = active ? y0 : 0.0;
-    nfe = nsetups = nje = nni = netf = ncfn = nfQe = nniS = 0;
+    nfe = nsetups = nje = nni = netf = ncfn = nfQe = nniS = nor = 0;
     dsm_last = 0.0; dsm_test = 0.0;
     ewt = 0.0; acor = 0.0; y = active ? y0 : 0.0; ftemp = 0.0;
 #pragma unroll
@@ -1222,6 +1229,56 @@ struct BDF {
     }
   }
 
+  // cvBDFStab (cvodes.c:8158-8219): at order q >= 3 the squared scaled
+  // derivative norms of orders q-1, q and q+1 of this step are pushed onto the
+  // five-step history; when no order increase is planned and q + 5 steps have
+  // passed since the last change of step size (nscon), the detector decides
+  // whether the order is stability-limited, and if so the next step is taken
+  // at order q-1 with that order's step ratio.  An order increase restarts the
+  // count.
+  __device__ __forceinline__ void bdf_stab() {
+    if (q >= 3) {
+      double fact = 1.0;
+      for (int i = 2; i < q; ++i) fact *= i;
+      const double sq = fact * q * (q + 1) * acnrm / fmax(S.tq[5], 1.0e-10);
+      const double sqm1 = fact * q * wrms(Z(q), ewt);
+      const double sqm2 = fact * wrms(Z(q - 1), ewt);
+      double ss[4][3];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) ss[i][k] = S.ssdat[i][k];
+      team_sync();
+      if (tm.lane == 0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int k = 0; k < 3; ++k) S.ssdat[i + 1][k] = ss[i][k];
+        S.ssdat[0][0] = sqm2 * sqm2;
+        S.ssdat[0][1] = sqm1 * sqm1;
+        S.ssdat[0][2] = sq * sq;
+      }
+      team_sync();
+    }
+    if (qprime >= q) {
+      if (q >= 3 && nscon >= q + 5) {
+        double ss[5][3];
+#pragma unroll
+        for (int i = 0; i < 5; ++i)
+#pragma unroll
+          for (int k = 0; k < 3; ++k) ss[i][k] = S.ssdat[i][k];
+        if (stald_flag(ss, q) > 3) {
+          qprime = q - 1;
+          eta = fmin(etaqm1, etamax);
+          hprime = h * eta;
+          nor++;
+        }
+      }
+    } else {
+      nscon = 0;
+    }
+  }
+
   // ---------------- one step: cvStep ----------------
   __device__ __forceinline__ int step() {
     int ncf = 0, nef = 0, nefQ = 0, ncfS = 0, nefS = 0;
@@ -1288,6 +1345,7 @@ struct BDF {
     record(1, dsm);
     AMX_PF_BEGIN(t7);
     prepare_next_step(dsm);
+    if (sp.stldet) bdf_stab();
     AMX_PF_END(t7, PF_PREPARE);
     etamax = (nst <= SMALL_NST) ? ETA_MAX_ES : ETA_MAX_GS;
     dsm_last = dsm;

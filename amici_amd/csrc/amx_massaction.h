@@ -61,8 +61,9 @@ struct BatchArgs {
 };
 
 // stats layout: forward nst, nfe, nsetups, nje, nni, netf, ncfn, last order,
-//               backward nstB, nfeB, nsetupsB, njeB, nniB, netfB, ncfnB, nfQeB
-constexpr int AMX_NSTATS = 16;
+//               backward nstB, nfeB, nsetupsB, njeB, nniB, netfB, ncfnB, nfQeB,
+//               order reductions by stability-limit detection forward, backward
+constexpr int AMX_NSTATS = 18;
 // per-output-time diagnostics: forward numsteps, rhs evals, error-test
 // failures, nonlinear-solver convergence failures, last order; backward
 // numsteps, rhs evals, error-test failures, convergence failures
@@ -81,7 +82,7 @@ struct Smem {
   static constexpr int JLD = M::NX + 1;
   static constexpr int NV = (M::NR > M::NY ? M::NR : M::NY) > M::NX
                                 ? (M::NR > M::NY ? M::NR : M::NY) : M::NX;
-  double x[M::NX];
+  double x[M::NX + 1];  // x[NX] = 1: the factor of unimolecular table entries
   double v[M::NX];
   double pk[M::NR];   // rate constant of every reaction
   double w[NV];       // fluxes / monomials / observable scratch
@@ -138,9 +139,11 @@ struct MAEval {
     }
     team_sync();
     double f = 0.0;
-    if (tm.lane < NX)
-      for (int q = M::SP_PTR[tm.lane]; q < M::SP_PTR[tm.lane + 1]; ++q)
-        f = fma(M::SP_C[q], s.w[M::SP_R[q]], f);
+#pragma unroll
+    for (int q = 0; q < M::SPL_N; ++q) {
+      const int k = q * T + tm.lane;
+      f = fma(M::SPLC[k], s.w[M::SPL[k]], f);
+    }
     return f;
   }
   // dense J into s.J (row-major, leading dim JLD), or for big models its
@@ -181,13 +184,22 @@ struct MAEval {
     }
     __syncthreads();
   }
-  // u_r = sum_i S_ir v_i into s.u; v in s.v
+  // u_r = sum_i S_ir v_i into s.u; v in s.v.  The per-lane sums of this and
+  // the other model evaluations run over the lane-major padded tables (entry q
+  // of all lanes in one coalesced row, fixed trip counts, fully unrolled), so
+  // every table load of a sum is issued at once.
   __device__ __forceinline__ void stoich_t() {
-    for (int r = tm.lane; r < NR; r += T) {
+    constexpr int ER = (NR + T - 1) / T;
+#pragma unroll
+    for (int e = 0; e < ER; ++e) {
+      const int r = tm.lane + e * T;
       double u = 0.0;
-      for (int q = M::RS_PTR[r]; q < M::RS_PTR[r + 1]; ++q)
-        u = fma(M::RS_C[q], s.v[M::RS_S[q]], u);
-      s.u[r] = u;
+#pragma unroll
+      for (int q = 0; q < M::RSL_N; ++q) {
+        const int k = (e * M::RSL_N + q) * T + tm.lane;
+        u = fma(M::RSLC[k], s.v[M::RSL[k]], u);
+      }
+      if (r < NR) s.u[r] = u;
     }
     team_sync();
   }
@@ -281,9 +293,11 @@ struct FwdProblem {
     }
     team_sync();
     double acc = 0.0;
-    if (tm.lane < NX)
-      for (int q = M::SP_PTR[tm.lane]; q < M::SP_PTR[tm.lane + 1]; ++q)
-        acc = fma(M::SP_C[q], s.u[M::SP_R[q]], acc);
+#pragma unroll
+    for (int q = 0; q < M::SPL_N; ++q) {
+      const int k = q * T + tm.lane;
+      acc = fma(M::SPLC[k], s.u[M::SPL[k]], acc);
+    }
     return acc;
   }
   __device__ __forceinline__ double sens_norm_slot(int slot) const {
@@ -669,13 +683,13 @@ struct BwdProblem {
   __device__ __forceinline__ double rhs(double t, double yB) {
     stage(t, yB);
     double f = 0.0;
-    if (tm.lane < NX)
-      for (int q = M::DX_PTR[tm.lane]; q < M::DX_PTR[tm.lane + 1]; ++q) {
-        const int r = M::DX_R[q], o = M::DX_O[q];
-        double d = M::DX_C[q] * s.pk[r];
-        if (o >= 0) d *= s.x[o];
-        f = fma(-d, s.u[r], f);
-      }
+#pragma unroll
+    for (int q = 0; q < M::DXL_N; ++q) {
+      const int k = q * T + tm.lane;
+      const unsigned e = M::DXL[k];
+      const int r = e & 0xFFFF, o = e >> 16;
+      f = fma(-M::DXLC[k] * s.pk[r] * s.x[o], s.u[r], f);
+    }
     return f;
   }
 
@@ -684,13 +698,13 @@ struct BwdProblem {
     ev.monomials();
 #pragma unroll
     for (int e = 0; e < EPQ; ++e) {
-      const int j = tm.lane + e * T;
       double acc = 0.0;
-      if (j < NP)
-        for (int q = M::PR_PTR[j]; q < M::PR_PTR[j + 1]; ++q) {
-          const int r = M::PR_R[q];
-          acc = fma(-s.w[r], s.u[r], acc);
-        }
+#pragma unroll
+      for (int q = 0; q < M::PRL_N; ++q) {
+        const int k = (e * M::PRL_N + q) * T + tm.lane;
+        const int r = M::PRL[k];
+        acc = fma(-M::PRLC[k] * s.w[r], s.u[r], acc);
+      }
       fq[e] = acc;
     }
   }
@@ -775,6 +789,7 @@ __device__ __forceinline__ void store_stats(int* st, const B& b, int off) {
   st[off + 5] = b.netf;
   st[off + 6] = b.ncfn;
   st[off + 7] = (off == 0) ? b.q : b.nfQe;
+  st[16 + (off == 0 ? 0 : 1)] = b.nor;
 }
 
 // ---------------------------------------------------------------------------
@@ -920,6 +935,7 @@ __device__ __forceinline__ void simulate_forward(int inst, const BatchArgs& a, c
   constexpr int NS = (MODE == 1) ? NP : 0;
   const size_t I = (size_t)inst;
   for (int r = tm.lane; r < M::NR; r += T) s.pk[r] = exp10(a.theta[I * NP + M::KI[r]]);
+  if (tm.lane == 0) s.x[M::NX] = 1.0;
   team_sync();
   double x0 = 0.0;
   if (tm.lane < NX) {
@@ -1091,6 +1107,7 @@ __device__ __forceinline__ void simulate_backward(int inst, const BatchArgs& a, 
     return;
   }
   for (int r = tm.lane; r < M::NR; r += T) s.pk[r] = exp10(a.theta[I * NP + M::KI[r]]);
+  if (tm.lane == 0) s.x[M::NX] = 1.0;
   team_sync();
   const int n = a.traj_n[inst];
   const double* tt = a.traj_t + I * (size_t)a.max_traj;

@@ -212,6 +212,7 @@ class Solver:
     rdata_reporting: RDataReporting = RDataReporting.full
     max_blocks: int = 0  # launch shape: 0 = as many blocks as fit the device at once
     interp: int = 2      # InterpolationType: 1 hermite, 2 polynomial (AMICI's default)
+    stldet: bool = True  # BDF stability-limit detection (AMICI's default: on)
 
     def set_relative_tolerance(self, v): self.rtol = float(v)
     def get_relative_tolerance(self): return self.rtol
@@ -254,6 +255,15 @@ class Solver:
     def get_interpolation_type(self):
         return self.interp
 
+    def set_stability_limit_flag(self, v):
+        """BDF stability-limit detection in the forward and backward problems
+        (Solver::set_stability_limit_flag, include/amici/solver.h:664-673; on
+        by default)."""
+        self.stldet = bool(v)
+
+    def get_stability_limit_flag(self):
+        return self.stldet
+
     def set_max_blocks(self, v):
         """Caps the blocks of the persistent kernels (no counterpart in the
         reference): with fewer blocks than instances each block takes further
@@ -279,7 +289,8 @@ class Solver:
                               maxstepsB=self.maxstepsB, max_traj=self.max_traj,
                               maxtime=self.maxtime, newton_maxsteps=self.newton_maxsteps,
                               ss_rtol=self.ss_rtol, ss_atol=self.ss_atol,
-                              max_blocks=self.max_blocks, interp=self.interp)
+                              max_blocks=self.max_blocks, interp=self.interp,
+                              stldet=self.stldet)
 
 
 class ExpData:

@@ -26,8 +26,8 @@
 extern "C" {
 #endif
 
-#define AMX_ABI_VERSION 4
-#define AMX_NSTATS 16
+#define AMX_ABI_VERSION 5
+#define AMX_NSTATS 18
 #define AMX_NDIAG 9
 /* bytes of the static workspace of amx_run_batch_static, per GPU */
 #define AMX_STATIC_WORKSPACE_BYTES (64u << 20)
@@ -36,6 +36,10 @@ extern "C" {
  * (amici::InterpolationType, include/amici/defines.h; CV_HERMITE / CV_POLYNOMIAL) */
 #define AMX_INTERP_HERMITE 1
 #define AMX_INTERP_POLYNOMIAL 2
+
+/* BDF stability-limit detection (Solver::setStabilityLimitFlag, CVodeSetStabLimDet) */
+#define AMX_STLDET_ON 1
+#define AMX_STLDET_OFF 2
 
 /* sensitivity method (amici::SensitivityMethod, include/amici/defines.h) */
 #define AMX_SENSI_NONE 0
@@ -63,6 +67,9 @@ typedef struct {
                                   reference: a launch-shape control)                    */
     int interp;                /* adjoint: AMX_INTERP_POLYNOMIAL (AMICI's default; 0 means
                                   the same) or AMX_INTERP_HERMITE (Solver::setInterpolationType) */
+    int stldet;                /* AMX_STLDET_ON (AMICI's default, include/amici/solver.h:2037;
+                                  0 means the same) or AMX_STLDET_OFF: BDF stability-limit
+                                  detection in the forward and backward problems */
 } amx_solver_opts;
 
 typedef struct {

@@ -52,7 +52,8 @@ typedef struct {
 } amx_oracle_opts;
 
 /* Counters: [0]nst [1]nfe [2]nsetups [3]nje [4]nni [5]netf [6]ncfn
- *           [7]nfSe [8]nniS [9]nstB [10]nfeB [11]nsetupsB [12]nniB [13]nfQeB [14]njeB */
+ *           [7]nfSe [8]nniS [9]nstB [10]nfeB [11]nsetupsB [12]nniB [13]nfQeB [14]njeB
+ *           [15]nor (forward order reductions by stability-limit detection) */
 #define AMX_ORACLE_NCOUNTERS 16
 
 /* One condition.  theta: [np] log10 parameters; kfix: [nk]; kpre: [nk]
@@ -86,6 +87,9 @@ void amx_oracle_dxdotdp(const amx_oracle_model* m, const double* p,
 int amx_oracle_trace_forward(const amx_oracle_model* m, const amx_oracle_opts* o,
                              const double* theta, const double* kfix, int nt, const double* ts,
                              double* rec, int cap);
+/* Called with the CVODES memory after every step of amx_oracle_trace_forward
+ * when set (used by stald_probe.c to read the stability-limit data). */
+extern void (*amx_oracle_step_hook)(void* cvode_mem);
 double amx_oracle_Jy(int noise, double y, double sigma, double m);
 double amx_oracle_dJydy(int noise, double y, double sigma, double m);
 

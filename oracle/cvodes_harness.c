@@ -496,6 +496,7 @@ int amx_oracle_simulate(const amx_oracle_model* m, const amx_oracle_opts* o,
         CVodeGetNumNonlinSolvIters(cv, &v); counters[4] = v;
         CVodeGetNumErrTestFails(cv, &v); counters[5] = v;
         CVodeGetNumNonlinSolvConvFails(cv, &v); counters[6] = v;
+        CVodeGetNumStabLimOrderReds(cv, &v); counters[15] = v;
         if (sx) {
             CVodeGetSensNumRhsEvals(cv, &v); counters[7] = v;
             if (o->ism != CV_SIMULTANEOUS) { CVodeGetSensNumNonlinSolvIters(cv, &v); counters[8] = v; }
@@ -651,6 +652,8 @@ int amx_oracle_simulate_batch(const amx_oracle_model* m, const amx_oracle_opts* 
  * as CV_NORMAL mode and returns after each.  Every accepted step is recorded
  * as 9 doubles: t, step size used, order used, nst, nfe, nsetups, nje, netf,
  * ncfn.  Returns the number of records (at most cap) or a negative status. */
+void (*amx_oracle_step_hook)(void* cvode_mem) = NULL;
+
 int amx_oracle_trace_forward(const amx_oracle_model* m, const amx_oracle_opts* o,
                              const double* theta, const double* kfix, int nt, const double* ts,
                              double* rec, int cap) {
@@ -717,6 +720,7 @@ int amx_oracle_trace_forward(const amx_oracle_model* m, const amx_oracle_opts* o
                     CVodeGetNumErrTestFails(cv, &v); r[7] = (double)v;
                     CVodeGetNumNonlinSolvConvFails(cv, &v); r[8] = (double)v;
                 }
+                if (amx_oracle_step_hook) amx_oracle_step_hook(cv);
                 ++n;
             }
         }

@@ -161,6 +161,51 @@ def trace_forward(network, theta, kfix, ts, *, rtol=1e-8, atol=1e-12, stldet=1, 
     return rec[:min(n, cap)]
 
 
+STALD_LIB = HERE / "_ref" / "libamx_stald_ref.so"
+_stald = None
+
+
+def _stald_lib() -> ctypes.CDLL:
+    global _stald
+    if _stald is None:
+        if not STALD_LIB.exists():
+            if Path("/root/reference/ThirdParty/sundials").exists():
+                build()
+            else:
+                raise FileNotFoundError(f"{STALD_LIB} missing and the reference is not available")
+        L = ctypes.CDLL(str(STALD_LIB))
+        L.amx_oracle_sldet.restype = ctypes.c_int
+        L.amx_oracle_sldet.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.amx_oracle_stald_samples.restype = ctypes.c_int
+        _stald = L
+    return _stald
+
+
+def sldet(ss, q: int) -> int:
+    """CVODES's stability-limit detector (cvSLdet) on ss [5][3] (newest step
+    first; orders q-1, q, q+1 of the squared scaled derivative norms)."""
+    ss = np.ascontiguousarray(ss, np.float64).reshape(5, 3)
+    return int(_stald_lib().amx_oracle_sldet(_p(ss), int(q)))
+
+
+def stald_samples(network, theta, kfix, ts, *, rtol=1e-8, atol=1e-12, cap=50000) -> np.ndarray:
+    """The detector inputs of one forward integration with stability-limit
+    detection on: rows (q, ss[5][3] flattened) of every step at which CVODES
+    called the detector."""
+    om = OracleModel(network)
+    theta = np.ascontiguousarray(theta, np.float64)
+    kfix = np.ascontiguousarray(np.asarray(kfix, np.float64).reshape(network.nk))
+    ts = np.ascontiguousarray(ts, np.float64)
+    out = np.zeros((cap, 16))
+    opts = _Opts(rtol, atol, 1e-8, 1e-12, 100000, 0, CV_SIMULTANEOUS, CV_POLYNOMIAL, 1, 0.0)
+    n = _stald_lib().amx_oracle_stald_samples(
+        ctypes.byref(om.c), ctypes.byref(opts), _p(theta), _p(kfix), ctypes.c_int(len(ts)),
+        _p(ts), _p(out), ctypes.c_int(cap))
+    if n < 0:
+        raise RuntimeError(f"oracle integration failed ({n})")
+    return out[:min(n, cap)]
+
+
 def Jy(noise: int, y: float, sigma: float, m: float) -> float:
     return lib().amx_oracle_Jy(noise, y, sigma, m)
 

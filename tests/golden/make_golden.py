@@ -185,6 +185,21 @@ def n40_counters() -> None:
                         llh=r["llh"])
 
 
+def n40_stald() -> None:
+    """Forward solver counters of all 1024 N40 conditions (states only, rtol
+    1e-8 / atol 1e-12) with stability-limit detection on (AMICI's default)
+    and off: steps and order reductions by the detector.  The inputs are
+    networks.condition_set("N40"), identified by the sum of its theta."""
+    cs = networks.condition_set("N40")
+    net, theta, kfix, ts = cs["network"], cs["theta"], cs["kfix"], cs["ts"]
+    on = O.simulate(net, theta, kfix, ts, nthreads=8, stldet=1, **BENCH)
+    off = O.simulate(net, theta, kfix, ts, nthreads=8, stldet=0, **BENCH)
+    assert (on["status"] == 0).all() and (off["status"] == 0).all()
+    np.savez_compressed(OUT / "n40_stald1024.npz", theta_sum=theta.sum(),
+                        counters_on=on["counters"].astype(np.int32),
+                        counters_off=off["counters"].astype(np.int32))
+
+
 def n200() -> None:
     """4 conditions of the 200-species network: states, observables, llh and
     the adjoint gradient at rtol 1e-10 / atol 1e-14, plus the rtol-1e-8
@@ -245,6 +260,8 @@ def step_limit() -> None:
 if __name__ == "__main__":
     if sys.argv[1:] == ["n40_counters"]:
         n40_counters()
+    elif sys.argv[1:] == ["n40_stald"]:
+        n40_stald()
     elif sys.argv[1:] == ["n40_bench_accuracy"]:
         n40_bench_accuracy()
     elif sys.argv[1:] == ["robertson_mix"]:

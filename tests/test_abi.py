@@ -163,15 +163,31 @@ def test_teams_narrower_than_a_wavefront_are_refused():
         codegen.emit_model_traits(networks.network_n40(), 32)
 
 
-def test_abi_v4_struct_layout():
-    """amx_solver_opts ends with max_blocks and interp, amx_batch_out with diag
-    (include/amx.h, ABI version 4)."""
-    assert _lib.AMX_ABI_VERSION == 4
-    assert [f[0] for f in _lib.SolverOpts._fields_[-2:]] == ["max_blocks", "interp"]
+def test_abi_v5_struct_layout():
+    """amx_solver_opts ends with max_blocks, interp and stldet, amx_batch_out
+    with diag; 18 counters per instance (include/amx.h, ABI version 5)."""
+    assert _lib.AMX_ABI_VERSION == 5 and _lib.AMX_NSTATS == 18
+    assert [f[0] for f in _lib.SolverOpts._fields_[-3:]] == ["max_blocks", "interp", "stldet"]
     assert _lib.BatchOut._fields_[-1][0] == "diag"
     text = (ROOT / "include" / "amx.h").read_text()
-    assert "#define AMX_ABI_VERSION 4" in text and "#define AMX_NDIAG 9" in text
+    assert "#define AMX_ABI_VERSION 5" in text and "#define AMX_NDIAG 9" in text
+    assert "#define AMX_NSTATS 18" in text
     assert "#define AMX_INTERP_HERMITE 1" in text and "#define AMX_INTERP_POLYNOMIAL 2" in text
+    assert "#define AMX_STLDET_ON 1" in text and "#define AMX_STLDET_OFF 2" in text
+    assert _lib.make_opts().stldet == _lib.STLDET_ON
+    assert _lib.make_opts(stldet=False).stldet == _lib.STLDET_OFF
+
+
+def test_invalid_stldet_rejected(lib):
+    net, L = lib
+    buf = (ctypes.c_double * 4)()
+    p = ctypes.cast(buf, ctypes.c_void_p).value
+    ins = _lib.BatchIn(1, 3, 0, p, p if net.nk else None, p, None, p, None)
+    outs = _lib.BatchOut(None, None, None, p, p, p, None, None)
+    o = _lib.make_opts()
+    o.stldet = 3
+    assert L.lib.amx_run_batch(ctypes.byref(o), ctypes.byref(ins), ctypes.byref(outs),
+                               p, 8, None) == -22
 
 
 def test_interpolation_workspace(lib):

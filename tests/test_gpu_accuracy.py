@@ -185,3 +185,36 @@ def test_per_output_time_diagnostics(gpu):
         assert rd.numsteps[-1] == rd.stats[0] and rd.num_rhs_evals[-1] == rd.stats[1]
         assert 1 <= rd.order[-1] <= 5
         assert rd.numsteps_b[0] == 0 and 0 < rd.numsteps_b[-1] <= rd.stats[8]
+
+
+def test_stability_limit_detection(gpu):
+    """BDF stability-limit detection (cvBDFStab / cvSLdet, AMICI's default)
+    in the forward integration of all 1024 N40 conditions at the benchmark's
+    tolerances, against CVODES's counters (fixture n40_stald1024): with the
+    detector on, the total steps within 0.3 % and the order reductions it
+    causes within 15 % of CVODES's; with it off, no reductions and the total
+    steps within 0.3 % of CVODES's without the detector."""
+    import torch
+    from amici_amd import sim
+    g = dict(np.load(GOLD / "n40_stald1024.npz"))
+    cs = networks.condition_set("N40")
+    assert np.isclose(cs["theta"].sum(), g["theta_sum"], rtol=1e-12, atol=0)
+    model = sim.Model(cs["network"])
+    for flag, key in ((True, "on"), (False, "off")):
+        sol = _solver()
+        sol.set_stability_limit_flag(flag)
+        out = _np(sim.simulate_batch(model, sol, cs["theta"], cs["kfix"], cs["ts"], want_x=False,
+                                     want_y=False))
+        torch.cuda.synchronize()
+        assert (out["status"] == 0).all()
+        cv = g[f"counters_{key}"]
+        nst, nor = out["stats"][:, 0], out["stats"][:, 16]
+        r = nst.sum() / cv[:, 0].sum()
+        print(f"stldet {key}: steps GPU {nst.mean():.2f} CVODES {cv[:, 0].mean():.2f}; order "
+              f"reductions GPU {nor.sum()} CVODES {cv[:, 15].sum()}; equal step counts "
+              f"{(nst == cv[:, 0]).mean():.3f}")
+        assert abs(r - 1.0) < 3e-3, (key, r)
+        if flag:
+            assert abs(nor.sum() - cv[:, 15].sum()) <= 0.15 * cv[:, 15].sum(), (nor.sum(), cv[:, 15].sum())
+        else:
+            assert nor.sum() == 0

@@ -268,3 +268,46 @@ def test_adjoint_gradient_hermite(models, model, fixture):
     assert (out["status"] == 0).all(), out["status"]
     _check_llh(models[model].network, out, g)
     _close(out["sllh"], g["sllh_asa"], 1e-5, 1e-7, "sllh (adjoint, Hermite)")
+
+
+def test_mix_composition(gpu):
+    """The mixed workload's three model types at fixture size, run together
+    on concurrent streams (sim.simulate_concurrent): 8 Robertson conditions of
+    the mixed set (adjoint), 16 pre-equilibrated open 40-species conditions
+    (adjoint) and 4 pre-equilibrated open 200-species conditions (adjoint),
+    each group against its oracle fixture at the parity bar."""
+    from amici_amd import sim
+    gr, g40, g200 = _load("robertson_mix1024"), _load("n40open_preeq16"), _load("n200open_preeq4")
+    idx = np.nonzero(gr["status"] == 0)[0][:8]
+    nets = [networks.robertson(), networks.network_n40_open(), networks.network_n200_open()]
+
+    def solver(steady=False):
+        sol = sim.Solver(**TIGHT)
+        if steady:
+            sol.set_relative_tolerance_steady_state(1e-12)
+            sol.set_absolute_tolerance_steady_state(1e-16)
+        sol.set_sensitivity_order(sim.SensitivityOrder.first)
+        sol.set_sensitivity_method(sim.SensitivityMethod.adjoint)
+        return sol
+
+    nr = len(idx)
+    groups = [
+        dict(model=sim.Model(nets[0]), solver=solver(), theta=gr["theta"][idx], kfix=np.zeros((nr, 0)),
+             ts=gr["ts"], meas=gr["meas"][idx], sigma=np.full(gr["meas"][idx].shape, float(gr["sigma"]))),
+        dict(model=sim.Model(nets[1]), solver=solver(), theta=g40["theta"], kfix=g40["kfix"],
+             ts=g40["ts"], meas=g40["meas"], sigma=g40["sigma"], kfix_preeq=g40["kfix_preeq"]),
+        dict(model=sim.Model(nets[2]), solver=solver(steady=True), theta=g200["theta"],
+             kfix=g200["kfix"], ts=g200["ts"], meas=g200["meas"], sigma=g200["sigma"],
+             kfix_preeq=g200["kfix_preeq"]),
+    ]
+    outs = [{k: (v.cpu().numpy() if hasattr(v, "cpu") else v) for k, v in o.items()
+             if k != "_keepalive"} for o in sim.simulate_concurrent(groups)]
+    for o in outs:
+        assert (o["status"] == 0).all(), o["status"]
+    r, o40, o200 = outs
+    _close(r["sllh"], gr["sllh_asa"][idx], 1e-5, 1e-7, "Robertson sllh (adjoint)")
+    np.testing.assert_allclose(r["llh"], gr["llh"][idx], rtol=1e-6)
+    for o, g, net, what in ((o40, g40, nets[1], "n40open"), (o200, g200, nets[2], "n200open")):
+        _close(o["x"], g["x"], Y_RTOL, Y_FLOOR, f"{what} x")
+        _check_llh(net, o, g)
+        _close(o["sllh"], g["sllh_asa"], 1e-5, 1e-7, f"{what} sllh (adjoint)")
