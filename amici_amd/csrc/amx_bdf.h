@@ -1238,11 +1238,17 @@ struct BDF {
   // count.
   __device__ __forceinline__ void bdf_stab() {
     if (q >= 3) {
+      // the squares of sqm2 = (q-1)! |zn[q-1]|, sqm1 = q! |zn[q]| (WRMS
+      // norms) and sq = (q+1)! acnrm / tq[5], formed from the sums of squares
+      // without the square roots
       double fact = 1.0;
       for (int i = 2; i < q; ++i) fact *= i;
-      const double sq = fact * q * (q + 1) * acnrm / fmax(S.tq[5], 1.0e-10);
-      const double sqm1 = fact * q * wrms(Z(q), ewt);
-      const double sqm2 = fact * wrms(Z(q - 1), ewt);
+      const double fq = fact * q;
+      const double e1 = active ? Z(q) * ewt : 0.0;
+      const double e2 = active ? Z(q - 1) * ewt : 0.0;
+      const double s1 = tm.sum(e1 * e1), s2 = tm.sum(e2 * e2);
+      constexpr double RNX = 1.0 / NX;
+      const double sq = fq * (q + 1) * stald_div(acnrm, fmax(S.tq[5], 1.0e-10));
       double ss[4][3];
 #pragma unroll
       for (int i = 0; i < 4; ++i)
@@ -1254,8 +1260,8 @@ struct BDF {
         for (int i = 0; i < 4; ++i)
 #pragma unroll
           for (int k = 0; k < 3; ++k) S.ssdat[i + 1][k] = ss[i][k];
-        S.ssdat[0][0] = sqm2 * sqm2;
-        S.ssdat[0][1] = sqm1 * sqm1;
+        S.ssdat[0][0] = fact * fact * (s2 * RNX);
+        S.ssdat[0][1] = fq * fq * (s1 * RNX);
         S.ssdat[0][2] = sq * sq;
       }
       team_sync();
