@@ -9,10 +9,12 @@ import ctypes
 import math
 from pathlib import Path
 
-AMX_ABI_VERSION = 5
+AMX_ABI_VERSION = 6
 AMX_NSTATS = 18
 AMX_NDIAG = 9
 SENSI_NONE, SENSI_FORWARD, SENSI_ADJOINT = 0, 1, 2
+# ParameterScaling (include/amici/defines.h)
+PSCALE_NONE, PSCALE_LN, PSCALE_LOG10 = 0, 1, 2
 INTERP_HERMITE, INTERP_POLYNOMIAL = 1, 2
 STLDET_ON, STLDET_OFF = 1, 2
 
@@ -44,6 +46,8 @@ class BatchIn(ctypes.Structure):
         ("theta", ctypes.c_void_p), ("kfix", ctypes.c_void_p), ("ts", ctypes.c_void_p),
         ("meas", ctypes.c_void_p), ("sigma", ctypes.c_void_p),
         ("kfix_preeq", ctypes.c_void_p),
+        ("nplist", ctypes.c_int), ("plist", ctypes.c_void_p), ("pscale", ctypes.c_void_p),
+        ("x0", ctypes.c_void_p),
     ]
 
 

@@ -22,8 +22,10 @@ from . import sim
 def _graph_for(model, solver, theta, kfix, ts, meas, sigma) -> sim.BatchGraph:
     """The model's cached graph for this solver setting and input shapes
     (captured on first use), with the inputs copied in."""
+    x0 = model._x0
     key = (repr(sorted(vars(solver).items())), tuple(theta.shape), tuple(torch.as_tensor(ts).shape),
-           tuple(torch.as_tensor(meas).shape))
+           tuple(torch.as_tensor(meas).shape), tuple(model.get_parameter_scale()),
+           None if x0 is None else tuple(x0))
     cache = model.__dict__.setdefault("_nll_graphs", {})
     g = cache.get(key)
     if g is None:
@@ -66,13 +68,19 @@ def adjoint_solver(**kw) -> sim.Solver:
 
 def nll(theta: torch.Tensor, model: sim.Model, kfix, ts, meas, sigma,
         solver: sim.Solver | None = None, graph: bool = False) -> torch.Tensor:
-    """Per-condition NLL [nb] of log10 parameters ``theta`` [nb, np] (a
-    device tensor); differentiable w.r.t. ``theta``.  ``graph`` replays a
-    captured HIP graph of the batch (one per shape and solver setting)."""
+    """Per-condition NLL [nb] of the free parameters ``theta`` [nb, np] (a
+    device tensor, on the model's parameter scales, log10 by default);
+    differentiable w.r.t. ``theta``, which needs the sensitivities of every
+    parameter (the model's parameter list in its default order).  ``graph``
+    replays a captured HIP graph of the batch (one per shape and solver
+    setting)."""
     if solver is None:
         solver = adjoint_solver()
     elif solver.effective_method() != sim.SensitivityMethod.adjoint:
         raise ValueError("nll needs a solver with first-order adjoint sensitivities")
     if theta.dim() != 2 or theta.shape[1] != model.np:
         raise ValueError(f"theta must be [nb, {model.np}]")
+    if model.get_parameter_list() != list(range(model.np)):
+        raise ValueError("nll differentiates with respect to every parameter: reset the "
+                         "model's parameter list (Model.set_parameter_list(range(np)))")
     return NLL.apply(theta, model, solver, kfix, ts, meas, sigma, graph)

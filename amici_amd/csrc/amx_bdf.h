@@ -529,6 +529,7 @@ struct BDF {
   Team<T> tm;
   SolverParams sp;
   bool active;  // lane < NX
+  int nq;       // quadratures integrated (<= NQ; the problem's parameter list)
 
   // per-lane work values
   double ewt, acor, y, ftemp;
@@ -557,6 +558,8 @@ struct BDF {
 
   __device__ __forceinline__ BDF(P& p, L& lds, const SolverParams& s) : pb(p), S(lds), sp(s) {
     active = tm.lane < NX;
+    if constexpr (QUAD) nq = p.nq;
+    else nq = 0;
     trace = nullptr;
     trace_cap = 0;
     trace_n = 0;
@@ -605,10 +608,10 @@ struct BDF {
     double acc = 0.0;
 #pragma unroll
     for (int k = 0; k < EPQ; ++k) {
-      const double e = (qslot(k) < NQ) ? v[k] * w[k] : 0.0;
+      const double e = (qslot(k) < nq) ? v[k] * w[k] : 0.0;
       acc += e * e;
     }
-    return sqrt(tm.sum(acc) / (NQ > 0 ? NQ : 1));
+    return sqrt(tm.sum(acc) / (nq > 0 ? nq : 1));
   }
   // true when any lane holds a non-finite value (NaN-safe, team-uniform)
   __device__ __forceinline__ bool any_nonfinite(double v) const {
@@ -620,7 +623,7 @@ struct BDF {
   __device__ __forceinline__ void set_ewtQ() {
 #pragma unroll
     for (int k = 0; k < EPQ; ++k)
-      ewtQ[k] = (qslot(k) < NQ) ? 1.0 / (sp.rtolQ * fabs(ZQ(0, k)) + sp.atolQ) : 0.0;
+      ewtQ[k] = (qslot(k) < nq) ? 1.0 / (sp.rtolQ * fabs(ZQ(0, k)) + sp.atolQ) : 0.0;
   }
   __device__ __forceinline__ bool quad_errcon() const { return QUAD && sp.errconQ; }
   __device__ __forceinline__ void znQ0(double (&v)[EPQ]) const {
@@ -1367,7 +1370,7 @@ struct BDF {
         double m = 0.0;
 #pragma unroll
         for (int e = 0; e < EPQ; ++e)
-          if (qslot(e) < NQ)
+          if (qslot(e) < nq)
             m = fmax(m, fabs(ZQ(1, e)) / (HUB_FACTOR * fabs(ZQ(0, e)) + 1.0 / ewtQ[e]));
         hub_inv = fmax(hub_inv, tm.max(m));
       }

@@ -24,16 +24,20 @@ struct Launch;
 struct BatchArgs {
   int nb, nt;
   int ts_stride;           // 0: one time grid for all instances, else nt
-  const double* theta;     // [nb][NP] log10 parameters
+  const double* theta;     // [nb][NP] parameters on their scale (pscale)
   const double* kfix;      // [nb][NK]
   const double* ts;        // [nb or 1][nt]
   const double* meas;      // [nb][nt][NY] (NaN = missing) or null
   const double* sigma;     // [nb][nt][NY]
   double* x;               // [nb][nt][NX] or null
   double* y;               // [nb][nt][NY] or null
-  double* sx;              // [nb][nt][NP][NX] or null (FSA)
+  double* sx;              // [nb][nt][npl][NX] or null (FSA)
   double* llh;             // [nb]
-  double* sllh;            // [nb][NP] (FSA / adjoint)
+  double* sllh;            // [nb][npl] (FSA / adjoint)
+  const int* plist;        // [nplist] parameters of the sensitivities, or null (all, in order)
+  int nplist;
+  const int* pscale;       // [NP] ParameterScaling 0 none, 1 ln, 2 log10, or null (all log10)
+  const double* x0;        // [nb][NX] initial states replacing the model's, or null
   int* status;             // [nb]
   int* stats;              // [nb][AMX_NSTATS] or null
   int* diag;               // [nb][nt][AMX_NDIAG] per-output-time counters or null
@@ -58,7 +62,36 @@ struct BatchArgs {
   int trace_inst, trace_cap;
   SolverParams sp;         // forward
   SolverParams spB;        // backward (adjoint)
+
+  // sensitivities computed (the length of the parameter list)
+  __device__ __forceinline__ int npl(int np) const { return plist ? nplist : np; }
+  __device__ __forceinline__ int pidx(int i) const { return plist ? plist[i] : i; }
+  __device__ __forceinline__ int scale(int j) const { return pscale ? pscale[j] : 2; }
 };
+
+// unscale_parameters (src/misc.cpp) for ParameterScaling none / ln / log10,
+// and the chain-rule factor of ReturnData::apply_chain_rule_factor_to_
+// simulation_results (src/rdata.cpp:782-796): p ln 10, p, 1
+__device__ __forceinline__ double unscale_p(double v, int sc) {
+  return sc == 2 ? exp10(v) : (sc == 1 ? exp(v) : v);
+}
+__device__ __forceinline__ double chain_coeff(double p, int sc) {
+  return sc == 2 ? p * M_LN10 : (sc == 1 ? p : 1.0);
+}
+
+// The parameter list and scales are the caller's device arrays: an entry out
+// of range fails the instance with AMICI_ILL_INPUT instead of being used
+// (the reference raises on them in ConditionContext::apply_condition,
+// src/edata.cpp:446-458, and Model::set_parameter_list).
+template <class M, class TM>
+__device__ __forceinline__ bool parameters_valid(const BatchArgs& a, const TM& tm) {
+  bool bad = a.plist && (a.nplist < 0 || a.nplist > M::NP);
+  if (a.plist && !bad)
+    for (int i = tm.lane; i < a.nplist; i += M::T) bad |= a.plist[i] < 0 || a.plist[i] >= M::NP;
+  if (a.pscale)
+    for (int j = tm.lane; j < M::NP; j += M::T) bad |= a.pscale[j] < 0 || a.pscale[j] > 2;
+  return !tm.any(bad);
+}
 
 // stats layout: forward nst, nfe, nsetups, nje, nni, netf, ncfn, last order,
 //               backward nstB, nfeB, nsetupsB, njeB, nniB, netfB, ncfnB, nfQeB,
@@ -215,6 +248,11 @@ struct FwdProblem {
   Smem<M>& s;
   MAEval<M> ev;
   double* ws;  // FSA workspace of this instance: [SENS_SLOTS][NS][NX]
+  // sensitivities computed: ns <= NS, sensitivity is with respect to
+  // parameter pidx(is) (the parameter list; all parameters in order without one)
+  int ns = NS;
+  const int* plist = nullptr;
+  __device__ __forceinline__ int pidx(int is) const { return plist ? plist[is] : is; }
   // adjoint forward pass: Hermite data of every accepted step
   double* tt;
   double* ty;
@@ -276,9 +314,10 @@ struct FwdProblem {
     ev.put_x(y);
     ev.monomials();
   }
-  // sensitivity RHS for column is at the staged y:
-  //   (J v)_i + df_i/dp_is = sum_r S_ir (dw_r/dx . v + [KI(r) == is] m_r)
+  // sensitivity RHS for column is (parameter j = pidx(is)) at the staged y:
+  //   (J v)_i + df_i/dp_j = sum_r S_ir (dw_r/dx . v + [KI(r) == j] m_r)
   __device__ __forceinline__ double sens_f(double yS, int is) {
+    const int pj = pidx(is);
     team_sync();
     if (tm.lane < NX) s.v[tm.lane] = yS;
     team_sync();
@@ -288,7 +327,7 @@ struct FwdProblem {
       if (a >= 0) d = s.v[a] * ((b >= 0) ? s.x[b] : 1.0);
       if (b >= 0) d = fma(s.v[b], s.x[a], d);
       double u = s.pk[r] * d;
-      if (M::KI[r] == is) u += s.w[r];
+      if (M::KI[r] == pj) u += s.w[r];
       s.u[r] = u;
     }
     team_sync();
@@ -302,7 +341,7 @@ struct FwdProblem {
   }
   __device__ __forceinline__ double sens_norm_slot(int slot) const {
     double m = 0.0;
-    for (int is = 0; is < NS; ++is) {
+    for (int is = 0; is < ns; ++is) {
       const double e = (tm.lane < NX) ? S(slot, is) * S(EWT, is) : 0.0;
       m = fmax(m, sqrt(tm.sum(e * e) / NX));
     }
@@ -311,15 +350,15 @@ struct FwdProblem {
   __device__ __forceinline__ void sens_init() {
     if (tm.lane < NX)
       for (int sl = 0; sl < SENS_SLOTS; ++sl)
-        for (int is = 0; is < NS; ++is) S(sl, is) = 0.0;  // sx0 = dx0/dp = 0
+        for (int is = 0; is < ns; ++is) S(sl, is) = 0.0;  // sx0 = dx0/dp = 0
   }
   __device__ __forceinline__ void sens_set_ewt() {
     if (tm.lane < NX)
-      for (int is = 0; is < NS; ++is) S(EWT, is) = 1.0 / (ssp_rtol * fabs(S(0, is)) + ssp_atol);
+      for (int is = 0; is < ns; ++is) S(EWT, is) = 1.0 / (ssp_rtol * fabs(S(0, is)) + ssp_atol);
   }
   __device__ __forceinline__ void sens_predict(int q) {
     if (tm.lane >= NX) return;
-    for (int is = 0; is < NS; ++is) {
+    for (int is = 0; is < ns; ++is) {
       double z[LMAX];
 #pragma unroll
       for (int j = 0; j < LMAX; ++j) z[j] = (j <= q) ? S(j, is) : 0.0;
@@ -335,7 +374,7 @@ struct FwdProblem {
   }
   __device__ __forceinline__ void sens_restore(int q) {
     if (tm.lane >= NX) return;
-    for (int is = 0; is < NS; ++is) {
+    for (int is = 0; is < ns; ++is) {
       double z[LMAX];
 #pragma unroll
       for (int j = 0; j < LMAX; ++j) z[j] = (j <= q) ? S(j, is) : 0.0;
@@ -351,7 +390,7 @@ struct FwdProblem {
   }
   __device__ __forceinline__ void sens_rescale(int q, double eta) {
     if (tm.lane >= NX) return;
-    for (int is = 0; is < NS; ++is) {
+    for (int is = 0; is < ns; ++is) {
       double c = eta;
 #pragma unroll
       for (int j = 1; j <= QMAX; ++j) {
@@ -362,7 +401,7 @@ struct FwdProblem {
   }
   __device__ __forceinline__ void sens_increase(int q, double A1, const double (&ll)[LMAX]) {
     if (tm.lane >= NX) return;
-    for (int is = 0; is < NS; ++is) {
+    for (int is = 0; is < ns; ++is) {
       const double zL = A1 * S(QMAX, is);
 #pragma unroll
       for (int j = 2; j <= QMAX; ++j)
@@ -372,7 +411,7 @@ struct FwdProblem {
   }
   __device__ __forceinline__ void sens_decrease(int q, const double (&ll)[LMAX]) {
     if (tm.lane >= NX) return;
-    for (int is = 0; is < NS; ++is) {
+    for (int is = 0; is < ns; ++is) {
       const double zq = S(q, is);
 #pragma unroll
       for (int j = 2; j < QMAX; ++j)
@@ -381,7 +420,7 @@ struct FwdProblem {
   }
   __device__ __forceinline__ void sens_complete(int q, const double (&l)[LMAX]) {
     if (tm.lane >= NX) return;
-    for (int is = 0; is < NS; ++is) {
+    for (int is = 0; is < ns; ++is) {
       const double a = S(ACOR, is);
 #pragma unroll
       for (int j = 0; j <= QMAX; ++j)
@@ -390,12 +429,12 @@ struct FwdProblem {
   }
   __device__ __forceinline__ void sens_save_acor() {
     if (tm.lane >= NX) return;
-    for (int is = 0; is < NS; ++is) S(QMAX, is) = S(ACOR, is);
+    for (int is = 0; is < ns; ++is) S(QMAX, is) = S(ACOR, is);
   }
   __device__ __forceinline__ double sens_norm_zn(int j) { return sens_norm_slot(j); }
   __device__ __forceinline__ double sens_norm_qp1(double cquot) {
     double m = 0.0;
-    for (int is = 0; is < NS; ++is) {
+    for (int is = 0; is < ns; ++is) {
       const double e = (tm.lane < NX) ? (S(ACOR, is) - cquot * S(QMAX, is)) * S(EWT, is) : 0.0;
       m = fmax(m, sqrt(tm.sum(e * e) / NX));
     }
@@ -404,21 +443,21 @@ struct FwdProblem {
   __device__ __forceinline__ double sens_hub_inv() {
     double m = 0.0;
     if (tm.lane < NX)
-      for (int is = 0; is < NS; ++is)
+      for (int is = 0; is < ns; ++is)
         m = fmax(m, fabs(S(1, is)) / (0.1 * fabs(S(0, is)) + 1.0 / S(EWT, is)));
     return tm.max(m);
   }
   // znS[1] = fS(t, y, znS[0]) (unscaled)
   __device__ __forceinline__ void sens_init_deriv(double, double y, double) {
     sens_prepare(y);
-    for (int is = 0; is < NS; ++is) {
+    for (int is = 0; is < ns; ++is) {
       const double v = sens_f(tm.lane < NX ? S(0, is) : 0.0, is);
       if (tm.lane < NX) S(1, is) = v;
     }
   }
   __device__ __forceinline__ void sens_scale_deriv(double h) {
     if (tm.lane < NX)
-      for (int is = 0; is < NS; ++is) S(1, is) *= h;
+      for (int is = 0; is < ns; ++is) S(1, is) *= h;
   }
   __device__ __forceinline__ void sens_restart_q1(double t, double y, double f, double h) {
     sens_init_deriv(t, y, f);
@@ -429,7 +468,7 @@ struct FwdProblem {
   __device__ __forceinline__ double sens_ydd_norm(double, double yy, double, double hg) {
     sens_prepare(yy);
     double m = 0.0;
-    for (int is = 0; is < NS; ++is) {
+    for (int is = 0; is < ns; ++is) {
       const double ys = (tm.lane < NX) ? S(0, is) + hg * S(1, is) : 0.0;
       const double fs = sens_f(ys, is);
       const double e = (tm.lane < NX) ? (fs - S(1, is)) / hg * S(EWT, is) : 0.0;
@@ -446,7 +485,7 @@ struct FwdProblem {
     bool callSetup = false, jbad = false;
     for (;;) {
       if (tm.lane < NX)
-        for (int is = 0; is < NS; ++is) S(ACOR, is) = 0.0;
+        for (int is = 0; is < ns; ++is) S(ACOR, is) = 0.0;
       if (callSetup) {
         if (jbad) b.convfail = FAIL_BAD_J;
         const int r = b.lsetup(b.convfail);
@@ -458,7 +497,7 @@ struct FwdProblem {
       for (;;) {
         b.nniS++;
         double Del = 0.0;
-        for (int is = 0; is < NS; ++is) {
+        for (int is = 0; is < ns; ++is) {
           const double acS = (tm.lane < NX) ? S(ACOR, is) : 0.0;
           const double yS = (tm.lane < NX) ? S(0, is) + acS : 0.0;
           const double fS = sens_f(yS, is);
@@ -647,6 +686,11 @@ struct BwdProblem {
   TrajInterp<M, POLY> ip;
   double t_cached, x_cached;
   bool have_cached;
+  // quadratures integrated: nq <= NQ; quadrature slot j belongs to parameter
+  // pidx(j) (the parameter list; all parameters in order without one)
+  int nq = NQ;
+  const int* plist = nullptr;
+  __device__ __forceinline__ int pidx(int j) const { return plist ? plist[j] : j; }
 
   __device__ __forceinline__ bool on_step(double, double, double, double, int) { return true; }
 
@@ -698,14 +742,19 @@ struct BwdProblem {
     ev.monomials();
 #pragma unroll
     for (int e = 0; e < EPQ; ++e) {
+      // the entries of parameter pj in the parameter-lane tables: lane pj % T
+      // of pass pj / T (the lane itself when there is no parameter list)
+      const int j = tm.lane + e * T;
+      const int pj = pidx(j < nq ? j : 0);
+      const int base = (pj / T) * M::PRL_N * T + pj % T;
       double acc = 0.0;
 #pragma unroll
       for (int q = 0; q < M::PRL_N; ++q) {
-        const int k = (e * M::PRL_N + q) * T + tm.lane;
+        const int k = base + q * T;
         const int r = M::PRL[k];
         acc = fma(-M::PRLC[k] * s.w[r], s.u[r], acc);
       }
-      fq[e] = acc;
+      fq[e] = (j < nq) ? acc : 0.0;
     }
   }
 
@@ -934,30 +983,38 @@ __device__ __forceinline__ void simulate_forward(int inst, const BatchArgs& a, c
   constexpr int NX = M::NX, NP = M::NP, NK = M::NK, NY = M::NY, T = M::T;
   constexpr int NS = (MODE == 1) ? NP : 0;
   const size_t I = (size_t)inst;
-  for (int r = tm.lane; r < M::NR; r += T) s.pk[r] = exp10(a.theta[I * NP + M::KI[r]]);
+  for (int r = tm.lane; r < M::NR; r += T)
+    s.pk[r] = unscale_p(a.theta[I * NP + M::KI[r]], a.scale(M::KI[r]));
   if (tm.lane == 0) s.x[M::NX] = 1.0;
   team_sync();
+  // initial state: ExpData::x0 (Model::set_initial_state) or the model's
+  // (constants and fixed parameters)
   double x0 = 0.0;
   if (tm.lane < NX) {
     const int kx = M::X0K[tm.lane];
-    x0 = (kx >= 0) ? a.kfix[I * NK + kx] : M::X0C[tm.lane];
+    x0 = a.x0 ? a.x0[I * NX + tm.lane] : ((kx >= 0) ? a.kfix[I * NK + kx] : M::X0C[tm.lane]);
   }
   // pre-equilibration: steady state at the pre-equilibration inputs, then the
   // fixed-parameter-dependent initial states are reset from kfix
   // (reinitialize_fixed_parameter_initial_states)
-  int pre_status = AMX_SUCCESS;
+  int pre_status = parameters_valid<M>(a, tm) ? AMX_SUCCESS : AMX_ILL_INPUT;
   double xss = 0.0;
   constexpr bool BIG = big_model<M>();
   double* const mat = BIG ? a.lu_ws + I * (size_t)NX * NX : nullptr;
-  if (a.kfix_pre) {
+  if (a.kfix_pre && pre_status == AMX_SUCCESS) {
+    // from ExpData::x0 if given, else from the model's initial state at the
+    // pre-equilibration inputs (Model::initialize_states)
     double xp = 0.0;
     if (tm.lane < NX) {
       const int kx = M::X0K[tm.lane];
-      xp = (kx >= 0) ? a.kfix_pre[I * NK + kx] : M::X0C[tm.lane];
+      xp = (a.x0 || kx < 0) ? x0 : a.kfix_pre[I * NK + kx];
     }
     pre_status = steady_state<M>(a, tm, s, lds, xp, mat);
     xss = xp;
-    if (tm.lane < NX && M::X0K[tm.lane] < 0) x0 = xp;
+    if (tm.lane < NX) {
+      const int kx = M::X0K[tm.lane];
+      x0 = (kx >= 0) ? a.kfix[I * NK + kx] : xp;
+    }
     if constexpr (MODE == 2) {
       if (tm.lane < NX) a.xss[I * NX + tm.lane] = xp;
     }
@@ -965,6 +1022,9 @@ __device__ __forceinline__ void simulate_forward(int inst, const BatchArgs& a, c
   double* ws = (MODE == 1) ? a.sens_ws + I * (size_t)SENS_SLOTS * NP * NX : nullptr;
   using PB = FwdProblem<M, NS>;
   PB pb0(tm, s, ws);
+  pb0.ns = (MODE == 1) ? a.npl(NP) : 0;
+  pb0.plist = a.plist;
+  const int npl = a.npl(NP);
   pb0.ssp_rtol = a.sp.rtolS;
   pb0.ssp_atol = a.sp.atolS;
   if constexpr (MODE == 2) {
@@ -996,7 +1056,7 @@ __device__ __forceinline__ void simulate_forward(int inst, const BatchArgs& a, c
         status = AMX_SINGULAR_JACOBIAN;
       } else {
         pb.sens_prepare(xss);
-        for (int is = 0; is < NP; ++is) {
+        for (int is = 0; is < pb.ns; ++is) {
           const double fp = pb.sens_f(0.0, is);
           const double v = lu.solve(tm.lane < NX ? -fp : 0.0, tm, lds);
           if (tm.lane < NX) pb.S(0, is) = (M::X0K[tm.lane] >= 0) ? 0.0 : v;
@@ -1043,10 +1103,10 @@ __device__ __forceinline__ void simulate_forward(int inst, const BatchArgs& a, c
     }
     if constexpr (MODE == 1) {
       if (tm.lane < NX)
-        for (int is = 0; is < NP; ++is) {
+        for (int is = 0; is < pb.ns; ++is) {
           const double sxv = pb.S(0, is);
           pb.S(PB::ACC, is) -= dJdx * sxv;
-          if (a.sx) a.sx[((I * a.nt + it) * NP + is) * NX + tm.lane] = sxv;
+          if (a.sx) a.sx[((I * a.nt + it) * npl + is) * NX + tm.lane] = sxv;
         }
     }
   }
@@ -1072,16 +1132,19 @@ __device__ __forceinline__ void simulate_forward(int inst, const BatchArgs& a, c
     if constexpr (MODE == 2) a.traj_n[inst] = pb.ntraj;
   }
   if constexpr (MODE == 1) {
-    // sllh (log10 scale) and sx chain rule (rdata.cpp:782-866)
-    for (int is = 0; is < NP; ++is) {
+    // sllh and sx: chain rule of each parameter's scale on the non-zero
+    // entries (rdata.cpp:782-866)
+    for (int is = 0; is < pb.ns; ++is) {
       const double part = tm.sum(tm.lane < NX ? pb.S(PB::ACC, is) : 0.0);
-      const double pv = exp10(a.theta[I * NP + is]);
-      const double v = (status == AMX_SUCCESS) ? (part != 0.0 ? part * pv * M_LN10 : 0.0) : NAN;
-      if (tm.lane == 0) a.sllh[I * NP + is] = v;
+      const int pj = pb.pidx(is);
+      const int sc = a.scale(pj);
+      const double cf = chain_coeff(unscale_p(a.theta[I * NP + pj], sc), sc);
+      const double v = (status == AMX_SUCCESS) ? (part != 0.0 ? part * cf : 0.0) : NAN;
+      if (tm.lane == 0) a.sllh[I * npl + is] = v;
       if (a.sx && tm.lane < NX)
         for (int j = 0; j < a.nt; ++j) {
-          double& e = a.sx[((I * a.nt + j) * NP + is) * NX + tm.lane];
-          e = (j < it) ? (e != 0.0 ? e * pv * M_LN10 : 0.0) : NAN;
+          double& e = a.sx[((I * a.nt + j) * npl + is) * NX + tm.lane];
+          e = (j < it) ? (e != 0.0 ? e * cf : 0.0) : NAN;
         }
     }
   }
@@ -1101,12 +1164,14 @@ __device__ __forceinline__ void simulate_backward(int inst, const BatchArgs& a, 
   if (diag && tm.lane == 0)
     for (int j = 0; j < a.nt; ++j)
       for (int k = 5; k < AMX_NDIAG; ++k) diag[(size_t)j * AMX_NDIAG + k] = 0;
+  const int npl = a.npl(NP);
   const int fst = a.status[inst];
   if (fst != AMX_SUCCESS) {
-    for (int j = tm.lane; j < NP; j += T) a.sllh[I * NP + j] = NAN;
+    for (int j = tm.lane; j < npl; j += T) a.sllh[I * npl + j] = NAN;
     return;
   }
-  for (int r = tm.lane; r < M::NR; r += T) s.pk[r] = exp10(a.theta[I * NP + M::KI[r]]);
+  for (int r = tm.lane; r < M::NR; r += T)
+    s.pk[r] = unscale_p(a.theta[I * NP + M::KI[r]], a.scale(M::KI[r]));
   if (tm.lane == 0) s.x[M::NX] = 1.0;
   team_sync();
   const int n = a.traj_n[inst];
@@ -1114,6 +1179,8 @@ __device__ __forceinline__ void simulate_backward(int inst, const BatchArgs& a, 
   PB pb0(tm, s, tt, a.traj_y + I * (size_t)a.max_traj * NX,
          POLY ? a.traj_q + I * (size_t)a.max_traj : nullptr,
          POLY ? nullptr : a.traj_f + I * (size_t)a.max_traj * NX, n);
+  pb0.nq = npl;
+  pb0.plist = a.plist;
   constexpr bool BIG = big_model<M>();
   if constexpr (BIG) pb0.mat = a.lu_ws + I * (size_t)NX * NX;
   BDF<PB> bdf(pb0, lds, a.spB);  // holds pb0 and the LDS block by reference
@@ -1221,23 +1288,28 @@ __device__ __forceinline__ void simulate_backward(int inst, const BatchArgs& a, 
       for (int e = 0; e < EPQ; ++e) {
         const int j = tm.lane + e * T;
         double acc = 0.0;
-        if (j < NP)
-          for (int q = M::PR_PTR[j]; q < M::PR_PTR[j + 1]; ++q) {
+        if (j < npl) {
+          const int pj = pb0.pidx(j);
+          for (int q = M::PR_PTR[pj]; q < M::PR_PTR[pj + 1]; ++q) {
             const int r = M::PR_R[q];
             acc = fma(s.w[r], s.u[r], acc);
           }
+        }
         gss[e] = acc;
       }
     }
   }
-  // sllh = -(xB(t0).sx0 + qB) with sx0 = 0; log10 chain rule on non-zero entries
+  // sllh = -(xB(t0).sx0 + qB) with sx0 = 0; chain rule of each parameter's
+  // scale on the non-zero entries
 #pragma unroll
   for (int e = 0; e < EPQ; ++e) {
     const int j = tm.lane + e * T;
-    if (j < NP) {
+    if (j < npl) {
       const double g = gss[e] - qB[e];
-      const double pv = exp10(a.theta[I * NP + j]);
-      a.sllh[I * NP + j] = (status == AMX_SUCCESS) ? (g != 0.0 ? g * pv * M_LN10 : 0.0) : NAN;
+      const int pj = pb0.pidx(j);
+      const int sc = a.scale(pj);
+      const double cf = chain_coeff(unscale_p(a.theta[I * NP + pj], sc), sc);
+      a.sllh[I * npl + j] = (status == AMX_SUCCESS) ? (g != 0.0 ? g * cf : 0.0) : NAN;
     }
   }
 #ifdef AMX_PROFILE

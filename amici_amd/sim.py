@@ -64,6 +64,9 @@ class Model:
         self._ws: dict = {}
         self._sigma_res = False
         self._min_sigma = 50.0
+        self._plist = list(range(network.np))
+        self._pscale = [_lib.PSCALE_LOG10] * network.np
+        self._x0 = None
 
     # dimensions
     @property
@@ -85,7 +88,7 @@ class Model:
     ne = 0
 
     def nplist(self) -> int:
-        return self.network.np
+        return len(self._plist)
 
     # ids
     def get_state_ids(self):
@@ -101,12 +104,48 @@ class Model:
         return list(self.network.fixed_parameter_names)
 
     def get_parameter_list(self):
-        return list(range(self.np))
+        return list(self._plist)
+
+    def set_parameter_list(self, plist: Sequence[int]) -> None:
+        """Parameters whose sensitivities are computed (Model::set_parameter_list,
+        include/amici/model.h:954): sllh / sx columns follow this list."""
+        pl = [int(i) for i in plist]
+        if any(i < 0 or i >= self.np for i in pl):
+            raise ValueError(f"parameter indices must be in [0, {self.np})")
+        self._plist = pl
 
     def get_parameter_scale(self):
-        return ["log10"] * self.np
+        return list(self._pscale)
 
-    # values (free parameters in their log10 scale, as AMICI stores them)
+    def set_parameter_scale(self, pscale) -> None:
+        """Scale of every free parameter (Model::set_parameter_scale,
+        include/amici/model.h:414-423): one ParameterScaling for all or one per
+        parameter, 0 / "none", 1 / "ln", 2 / "log10"; the free parameters and the
+        sensitivities are on this scale."""
+        self._pscale = _pscale_list(pscale, self.np)
+
+    def set_initial_state(self, x0) -> None:
+        """Initial state replacing the model's (Model::set_initial_state);
+        an empty sequence restores the model's own."""
+        x0 = np.asarray(x0, np.float64).ravel()
+        if x0.size == 0:
+            self._x0 = None
+            return
+        if x0.shape != (self.nx_rdata,):
+            raise ValueError(f"expected {self.nx_rdata} initial states")
+        self._x0 = x0.copy()
+
+    def get_initial_state(self, kfix=None) -> np.ndarray:
+        """The initial state used: the one set, else the model's (constants
+        and fixed parameters)."""
+        if self._x0 is not None:
+            return self._x0.copy()
+        net = self.network
+        k = self._k if kfix is None else np.asarray(kfix, np.float64)
+        return np.array([k[j] if j >= 0 else c for j, c in zip(net.x0_fixed, net.x0_const)],
+                        np.float64)
+
+    # values (free parameters on their scale, as AMICI stores them)
     def set_free_parameters(self, theta: Sequence[float]) -> None:
         theta = np.asarray(theta, np.float64)
         if theta.shape != (self.np,):
@@ -296,8 +335,11 @@ class Solver:
 class ExpData:
     """Per-condition inputs (include/amici/edata.h): time grid, measurements
     [nt, ny] (NaN = missing), noise scales [nt, ny], fixed parameters, fixed
-    parameters of a pre-equilibration (empty = none) and optional free
-    parameters (log10 scale)."""
+    parameters of a pre-equilibration (empty = none), and the optional
+    overrides of the model's settings that ConditionContext::apply_condition
+    applies (src/edata.cpp:438-490): free parameters (on their scale),
+    parameter list ``plist``, parameter scales ``pscale`` and initial states
+    ``x0`` (empty = the model's)."""
 
     def __init__(self, source: "Model | ReturnData | None" = None, sigma_y=None,
                  sigma_z: float = 0.0, seed: int = -1):
@@ -315,6 +357,7 @@ class ExpData:
             self.fixed_parameters = np.zeros(0)
             self.free_parameters = np.zeros(0)
             self.fixed_parameters_pre_equilibration = np.zeros(0)
+            self.plist, self.pscale, self.x0 = [], [], np.zeros(0)
             nt = len(self.timepoints)
             sig = np.broadcast_to(np.asarray(1.0 if sigma_y is None else sigma_y, np.float64),
                                   (nt, self.ny)).copy()
@@ -331,6 +374,7 @@ class ExpData:
         self.fixed_parameters = model.get_fixed_parameters() if model else np.zeros(0)
         self.free_parameters = np.zeros(0)
         self.fixed_parameters_pre_equilibration = np.zeros(0)
+        self.plist, self.pscale, self.x0 = [], [], np.zeros(0)
         self._meas = np.full((len(self.timepoints), self.ny), np.nan)
         self._sigma = np.ones((len(self.timepoints), self.ny))
         self.id = ""
@@ -405,6 +449,27 @@ DIAG_FIELDS = ("numsteps", "num_rhs_evals", "num_err_test_fails", "num_non_lin_s
                "num_non_lin_solv_conv_fails_b")
 
 
+def _pscale_list(pscale, n: int) -> list:
+    names = {"none": _lib.PSCALE_NONE, "ln": _lib.PSCALE_LN, "log10": _lib.PSCALE_LOG10}
+
+    def one(v):
+        key = getattr(v, "name", v)
+        if isinstance(key, str):
+            if key.lower() not in names:
+                raise ValueError(f"unknown parameter scale {v!r}")
+            return names[key.lower()]
+        if int(v) not in (0, 1, 2):
+            raise ValueError(f"unknown parameter scale {v!r}")
+        return int(v)
+
+    if isinstance(pscale, (str, int)) or hasattr(pscale, "name"):
+        return [one(pscale)] * n
+    out = [one(v) for v in pscale]
+    if len(out) != n:
+        raise ValueError(f"expected {n} parameter scales")
+    return out
+
+
 def _as_dev(a, dev, dtype=torch.float64) -> torch.Tensor:
     if isinstance(a, torch.Tensor):
         return a.to(device=dev, dtype=dtype).contiguous()
@@ -416,11 +481,21 @@ def _ptr(t):
 
 
 def _prepare(model: Model, solver: Solver, theta, kfix, ts, meas, sigma, *, want_x, want_y,
-             want_sx, want_stats, kfix_preeq, copy_inputs=False) -> dict:
+             want_sx, want_stats, kfix_preeq, plist=None, pscale=None, x0=None,
+             copy_inputs=False) -> dict:
     """Device inputs, output tensors and the C structs of one batch call.
     With ``copy_inputs`` every input is a fresh device copy (for a graph that
-    keeps pointing at them)."""
+    keeps pointing at them).  plist / pscale default to the model's settings
+    (Model.set_parameter_list / set_parameter_scale), x0 [nb, nx] to the
+    model's initial state (Model.set_initial_state, else its own)."""
     dev = _device()
+    plist = model.get_parameter_list() if plist is None else [int(i) for i in plist]
+    pscale = model.get_parameter_scale() if pscale is None else _pscale_list(pscale, model.np)
+    if any(i < 0 or i >= model.np for i in plist):
+        raise ValueError(f"parameter indices must be in [0, {model.np})")
+    npl = len(plist)
+    if x0 is None and model._x0 is not None:
+        x0 = np.broadcast_to(model._x0, (np.shape(theta)[0], model.nx_rdata))
 
     def dev_t(a):
         t = _as_dev(a, dev)
@@ -437,6 +512,14 @@ def _prepare(model: Model, solver: Solver, theta, kfix, ts, meas, sigma, *, want
     sigma_t = dev_t(sigma)
     meas_t = dev_t(meas) if meas is not None else None
     kpre_t = dev_t(kfix_preeq) if kfix_preeq is not None else None
+    x0_t = dev_t(x0) if x0 is not None else None
+    if x0_t is not None and x0_t.shape != (nb, model.nx_rdata):
+        raise ValueError("x0 must be [nb, nx]")
+    # the identity list and all-log10 scales are the kernels' defaults (NULL)
+    plist_t = (None if plist == list(range(model.np))
+               else torch.tensor(plist, dtype=torch.int32, device=dev))
+    pscale_t = (None if all(v == _lib.PSCALE_LOG10 for v in pscale)
+                else torch.tensor(pscale, dtype=torch.int32, device=dev))
     if kpre_t is not None and kpre_t.shape != (nb, model.nk):
         raise ValueError("kfix_preeq must be [nb, nk]")
     if theta_t.shape != (nb, model.np) or kfix_t.shape != (nb, model.nk):
@@ -451,10 +534,10 @@ def _prepare(model: Model, solver: Solver, theta, kfix, ts, meas, sigma, *, want
     out = {
         "x": torch.empty((nb, nt, model.nx_rdata), **f64) if want_x else None,
         "y": torch.empty((nb, nt, model.ny), **f64) if want_y else None,
-        "sx": (torch.empty((nb, nt, model.np, model.nx_rdata), **f64)
+        "sx": (torch.empty((nb, nt, npl, model.nx_rdata), **f64)
                if (want_sx and meth == _lib.SENSI_FORWARD) else None),
         "llh": torch.empty(nb, **f64),
-        "sllh": torch.empty((nb, model.np), **f64) if meth != _lib.SENSI_NONE else None,
+        "sllh": torch.empty((nb, npl), **f64) if meth != _lib.SENSI_NONE else None,
         "status": torch.full((nb,), AMICI_NOT_RUN, dtype=torch.int32, device=dev),
         "stats": (torch.zeros((nb, _lib.AMX_NSTATS), dtype=torch.int32, device=dev)
                   if want_stats else None),
@@ -462,30 +545,35 @@ def _prepare(model: Model, solver: Solver, theta, kfix, ts, meas, sigma, *, want
                  if want_stats else None),
     }
     bin_ = _lib.BatchIn(nb, nt, ts_stride, _ptr(theta_t), _ptr(kfix_t) if model.nk else None,
-                        _ptr(ts_t), _ptr(meas_t), _ptr(sigma_t), _ptr(kpre_t))
+                        _ptr(ts_t), _ptr(meas_t), _ptr(sigma_t), _ptr(kpre_t),
+                        npl, _ptr(plist_t), _ptr(pscale_t), _ptr(x0_t))
     bout = _lib.BatchOut(*[_ptr(out[k]) for k in ("x", "y", "sx", "llh", "sllh", "status", "stats",
                                                   "diag")])
     inputs = dict(theta=theta_t, kfix=kfix_t, ts=ts_t, meas=meas_t, sigma=sigma_t,
-                  kfix_preeq=kpre_t)
+                  kfix_preeq=kpre_t, plist=plist_t, pscale=pscale_t, x0=x0_t)
     return dict(dev=dev, nb=nb, nt=nt, opts=opts, bin=bin_, bout=bout, out=out, inputs=inputs)
 
 
 def simulate_batch(model: Model, solver: Solver, theta, kfix, ts, meas=None, sigma=None, *,
                    want_x: bool = True, want_y: bool = True, want_sx: bool = False,
                    want_stats: bool = True, sequential: bool = False, stream=None,
-                   kfix_preeq=None) -> dict:
+                   kfix_preeq=None, plist=None, pscale=None, x0=None) -> dict:
     """Simulates ``nb`` instances in one batch; arrays may be numpy or torch.
 
-    theta [nb, np] (log10), kfix [nb, nk], ts [nt] or [nb, nt], meas / sigma
-    [nb, nt, ny].  Returns a dict of device tensors: x, y, sx, llh, sllh,
-    status, stats, diag.  ``sequential`` runs the plain one-instance-per-launch
-    path.  ``kfix_preeq`` [nb, nk] pre-equilibrates every instance to steady
-    state with these fixed parameters before the simulation.
+    theta [nb, np] (on the parameter scales), kfix [nb, nk], ts [nt] or
+    [nb, nt], meas / sigma [nb, nt, ny].  Returns a dict of device tensors: x,
+    y, sx, llh, sllh, status, stats, diag.  ``sequential`` runs the plain
+    one-instance-per-launch path.  ``kfix_preeq`` [nb, nk] pre-equilibrates
+    every instance to steady state with these fixed parameters before the
+    simulation.  ``plist`` (parameter indices; sllh [nb, len(plist)], sx [nb,
+    nt, len(plist), nx]), ``pscale`` (one ParameterScaling per parameter) and
+    ``x0`` [nb, nx] default to the model's settings.
     """
     import ctypes
 
     p = _prepare(model, solver, theta, kfix, ts, meas, sigma, want_x=want_x, want_y=want_y,
-                 want_sx=want_sx, want_stats=want_stats, kfix_preeq=kfix_preeq)
+                 want_sx=want_sx, want_stats=want_stats, kfix_preeq=kfix_preeq, plist=plist,
+                 pscale=pscale, x0=x0)
     lib, dev, opts = model.lib, p["dev"], p["opts"]
     nbytes = lib.workspace_bytes(opts, 1 if sequential else p["nb"], p["nt"])
     st = stream if stream is not None else torch.cuda.current_stream(dev)
@@ -497,7 +585,7 @@ def simulate_batch(model: Model, solver: Solver, theta, kfix, ts, meas=None, sig
         raise RuntimeError(f"amx_run_batch returned {rc}")
     out = p["out"]
     i = p["inputs"]
-    out["_keepalive"] = (i["theta"], i["kfix"], i["ts"], i["meas"], i["sigma"], i["kfix_preeq"])
+    out["_keepalive"] = tuple(i.values())
     return out
 
 
@@ -512,12 +600,12 @@ class BatchGraph:
 
     def __init__(self, model: Model, solver: Solver, theta, kfix, ts, meas=None, sigma=None, *,
                  want_x: bool = True, want_y: bool = True, want_sx: bool = False,
-                 want_stats: bool = True, kfix_preeq=None):
+                 want_stats: bool = True, kfix_preeq=None, plist=None, pscale=None, x0=None):
         import ctypes
 
         p = _prepare(model, solver, theta, kfix, ts, meas, sigma, want_x=want_x, want_y=want_y,
                      want_sx=want_sx, want_stats=want_stats, kfix_preeq=kfix_preeq,
-                     copy_inputs=True)
+                     plist=plist, pscale=pscale, x0=x0, copy_inputs=True)
         self.model = model
         self.inputs = p["inputs"]
         self.out = p["out"]
@@ -602,16 +690,26 @@ def run_simulations(model: Model, solver: Solver, edata_list: Sequence[ExpData],
     if len(edata_list) == 0:
         return []
     nb = len(edata_list)
-    nts = {len(e.timepoints) for e in edata_list}
-    if len(nts) != 1:
-        # group by time-grid length and reassemble in order
+
+    def key(e):
+        # one batch per time-grid length, parameter list and scales, and
+        # whether the initial state is given (ConditionContext::apply_condition,
+        # src/edata.cpp:438-490)
+        pl = tuple(int(i) for i in e.plist) if len(e.plist) else tuple(model.get_parameter_list())
+        ps = (tuple(_pscale_list(e.pscale, model.np)) if len(e.pscale)
+              else tuple(model.get_parameter_scale()))
+        return (len(e.timepoints), pl, ps, len(e.x0) > 0)
+
+    keys = [key(e) for e in edata_list]
+    if len(set(keys)) != 1:
         res = [None] * nb
-        for n in sorted(nts):
-            idx = [i for i, e in enumerate(edata_list) if len(e.timepoints) == n]
+        for k in sorted(set(keys)):
+            idx = [i for i, kk in enumerate(keys) if kk == k]
             sub = run_simulations(model, solver, [edata_list[i] for i in idx], failfast=False)
             for i, r in zip(idx, sub):
                 res[i] = r
         return _apply_failfast(res, failfast)
+    _, plist, pscale, has_x0 = keys[0]
     theta = np.stack([e.free_parameters if len(e.free_parameters) else model.get_free_parameters()
                       for e in edata_list])
     kfix = np.stack([np.asarray(e.fixed_parameters, np.float64) if len(e.fixed_parameters)
@@ -632,14 +730,17 @@ def run_simulations(model: Model, solver: Solver, edata_list: Sequence[ExpData],
     fsa = meth == _lib.SENSI_FORWARD
     fields = _rdata.reported_fields(solver.rdata_reporting, forward=fsa,
                                     sensi=meth != _lib.SENSI_NONE)
+    x0 = (np.stack([np.asarray(e.x0, np.float64).reshape(model.nx_rdata) for e in edata_list])
+          if has_x0 else None)
     out = simulate_batch(model, solver, theta, kfix, ts, meas, sigma, want_x="x" in fields,
-                         want_sx=fsa, kfix_preeq=kpre)
+                         want_sx=fsa, kfix_preeq=kpre, plist=list(plist), pscale=list(pscale),
+                         x0=x0)
     dev = out["llh"].device
     meas_t, sigma_t = out["_keepalive"][3], out["_keepalive"][4]
     sy_t = None
     if out["sx"] is not None:
         C = torch.as_tensor(model.network.observable_matrix, dtype=torch.float64, device=dev)
-        sy_t = out["sx"] @ C.T                                     # [nb, nt, np, ny]
+        sy_t = out["sx"] @ C.T                                     # [nb, nt, nplist, ny]
     derived = _rdata.derived_outputs(out["y"], sy_t, meas_t, sigma_t, out["status"],
                                      model.network.noise, sigma_res=model.get_add_sigma_residuals(),
                                      sigma_offset=model.get_minimum_sigma_residuals(), fields=fields)

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define AMX_ABI_VERSION 5
+#define AMX_ABI_VERSION 6
 #define AMX_NSTATS 18
 #define AMX_NDIAG 9
 /* bytes of the static workspace of amx_run_batch_static, per GPU */
@@ -76,7 +76,7 @@ typedef struct {
     int nb;                /* instances (conditions / parameter sets)                 */
     int nt;                /* output times per instance                               */
     int ts_stride;         /* 0: ts shared by all instances, nt: one row per instance */
-    const double* theta;   /* [nb][np] free parameters, log10 scale                   */
+    const double* theta;   /* [nb][np] free parameters on their scale (pscale; default log10) */
     const double* kfix;    /* [nb][nk] fixed parameters                               */
     const double* ts;      /* [1 or nb][nt] ascending, >= t0 = 0                      */
     const double* meas;    /* [nb][nt][ny] measurements, NaN = missing; may be NULL   */
@@ -85,14 +85,30 @@ typedef struct {
                                  state (ExpData::fixed_parameters_pre_equilibration), or NULL;
                                  after it the fixed-parameter-dependent initial states are
                                  reset from kfix                                         */
+    /* The following device arrays are optional (NULL: the default in brackets).
+     * plist, nplist: sensitivities with respect to parameters plist[0..nplist-1]
+     *   (Model::set_parameter_list, ExpData::plist; [all np in order]); sllh is
+     *   then [nb][nplist] and sx [nb][nt][nplist][nx].
+     * pscale: [np] ParameterScaling of each parameter, 0 none, 1 ln, 2 log10
+     *   (Model::set_parameter_scale, ExpData::pscale; [all log10]); theta is on
+     *   that scale and sllh / sx are derivatives with respect to it.
+     *   A list or scale entry out of range fails every instance with
+     *   AMICI_ILL_INPUT (-22).
+     * x0: [nb][nx] initial states replacing the model's (ExpData::x0,
+     *   Model::set_initial_state; [the model's]); a pre-equilibration starts
+     *   from it too.                                                          */
+    int nplist;
+    const int* plist;
+    const int* pscale;
+    const double* x0;
 } amx_batch_in;
 
 typedef struct {
     double* x;      /* [nb][nt][nx] or NULL                         */
     double* y;      /* [nb][nt][ny] or NULL                         */
-    double* sx;     /* [nb][nt][np][nx] or NULL (forward only)      */
+    double* sx;     /* [nb][nt][nplist][nx] or NULL (forward only)  */
     double* llh;    /* [nb] log-likelihood (required)               */
-    double* sllh;   /* [nb][np] gradient of llh w.r.t. log10 params */
+    double* sllh;   /* [nb][nplist] gradient of llh w.r.t. the scaled parameters */
     int* status;    /* [nb] AMICI status code per instance (required) */
     int* stats;     /* [nb][AMX_NSTATS] solver counters or NULL      */
     int* diag;      /* [nb][nt][AMX_NDIAG] per-output-time diagnostics or NULL

@@ -49,6 +49,11 @@ typedef struct {
     int interp;              /* CV_HERMITE=1, CV_POLYNOMIAL=2             */
     int stldet;              /* stability limit detection (AMICI default on) */
     double t_preeq;          /* pre-equilibration: integration end time */
+    int nplist;              /* sensitivities w.r.t. plist[0..nplist-1] (Model::set_parameter_list);
+                                plist NULL: all np parameters in order */
+    const int* plist;
+    const int* pscale;       /* [np] ParameterScaling (0 none, 1 ln, 2 log10) of theta, or NULL:
+                                all log10 (Model::set_parameter_scale) */
 } amx_oracle_opts;
 
 /* Counters: [0]nst [1]nfe [2]nsetups [3]nje [4]nni [5]netf [6]ncfn
@@ -56,14 +61,17 @@ typedef struct {
  *           [15]nor (forward order reductions by stability-limit detection) */
 #define AMX_ORACLE_NCOUNTERS 16
 
-/* One condition.  theta: [np] log10 parameters; kfix: [nk]; kpre: [nk]
- * fixed parameters of a pre-equilibration (NULL = none); ts: [nt];
+/* One condition.  theta: [np] parameters on their scale (pscale); kfix: [nk];
+ * kpre: [nk] fixed parameters of a pre-equilibration (NULL = none); x0: [nx]
+ * initial state replacing the model's (ExpData::x0, NULL = none); ts: [nt];
  * meas/sigma: [nt*ny] (NaN measurement = missing).
- * Outputs (may be NULL): x [nt*nx], y [nt*ny], sx [nt*np*nx] (FSA only),
- * llh [1], sllh [np] (log10 scale), counters [AMX_ORACLE_NCOUNTERS].
+ * Outputs (may be NULL): x [nt*nx], y [nt*ny], sx [nt*nplist*nx] (FSA only),
+ * llh [1], sllh [nplist] (w.r.t. the scaled parameters), counters
+ * [AMX_ORACLE_NCOUNTERS].
  * Returns AMICI status (0 success, <0 failure as in include/amici/defines.h). */
 int amx_oracle_simulate(const amx_oracle_model* m, const amx_oracle_opts* o,
-                        const double* theta, const double* kfix, const double* kpre, int nt,
+                        const double* theta, const double* kfix, const double* kpre,
+                        const double* x0, int nt,
                         const double* ts, const double* meas, const double* sigma,
                         double* x, double* y, double* sx, double* llh,
                         double* sllh, long* counters);
@@ -72,7 +80,8 @@ int amx_oracle_simulate(const amx_oracle_model* m, const amx_oracle_opts* o,
  * Per-condition arrays are strided by their single-condition size. */
 int amx_oracle_simulate_batch(const amx_oracle_model* m, const amx_oracle_opts* o,
                               int nb, const double* theta, const double* kfix,
-                              const double* kpre, int nt, const double* ts, const double* meas,
+                              const double* kpre, const double* x0, int nt, const double* ts,
+                              const double* meas,
                               const double* sigma, double* x, double* y,
                               double* sx, double* llh, double* sllh,
                               long* counters, int* status, int nthreads);

@@ -59,7 +59,17 @@ typedef struct {
     double* J;      /* [nx*nx] row-major              */
     double* dfdp;   /* [nx*np] row-major              */
     double* u;      /* [nr] S^T v scratch            */
+    const int* plist; /* sensitivity iS -> parameter plist[iS] (NULL: identity) */
+    double* qfull;  /* [np] adjoint quadrature integrand of every parameter */
 } ud_t;
+
+/* ParameterScaling (include/amici/defines.h): 0 none, 1 ln, 2 log10;
+ * unscale_parameters (src/misc.cpp) and the chain-rule factor of
+ * ReturnData::apply_chain_rule_factor_to_simulation_results
+ * (src/rdata.cpp:782-796: p ln 10, p, 1) */
+static int pscale_of(const amx_oracle_opts* o, int j) { return o->pscale ? o->pscale[j] : 2; }
+static double unscale(double v, int sc) { return sc == 2 ? pow(10.0, v) : (sc == 1 ? exp(v) : v); }
+static double chain_coeff(double p, int sc) { return sc == 2 ? p * log(10.0) : (sc == 1 ? p : 1.0); }
 
 /* ---------------- model pieces (mass action) ---------------- */
 
@@ -178,8 +188,9 @@ static int cb_fS1(int Ns, sunrealtype t, N_Vector x, N_Vector xdot, int iS,
     }
     const double* s = N_VGetArrayPointer(sx);
     double* o = N_VGetArrayPointer(sxdot);
+    const int ip = ud->plist ? ud->plist[iS] : iS;
     for (int i = 0; i < nx; ++i) {
-        double acc = ud->dfdp[i * np + iS];
+        double acc = ud->dfdp[i * np + ip];
         const double* Ji = ud->J + (size_t)i * nx;
         for (int j = 0; j < nx; ++j) acc += Ji[j] * s[j];
         o[i] = acc;
@@ -218,7 +229,7 @@ static int cb_fQB(sunrealtype t, N_Vector x, N_Vector xB, N_Vector qBdot, void* 
     const amx_oracle_model* m = ud->m;
     const double* xs = N_VGetArrayPointer(x);
     const double* b = N_VGetArrayPointer(xB);
-    double* o = N_VGetArrayPointer(qBdot);
+    double* o = ud->qfull;
     memset(o, 0, sizeof(double) * (size_t)m->np);
     for (int r = 0; r < m->nr; ++r) {
         double u = 0.0;
@@ -228,6 +239,10 @@ static int cb_fQB(sunrealtype t, N_Vector x, N_Vector xB, N_Vector qBdot, void* 
         for (int q = m->rptr[r]; q < m->rptr[r + 1]; ++q) prod *= xs[m->ridx[q]];
         o[m->kidx[r]] -= prod * u;
     }
+    /* the backward problem integrates nplist quadratures (model.h nplist) */
+    double* qd = N_VGetArrayPointer(qBdot);
+    const int nq = (int)N_VGetLength(qBdot);
+    for (int ip = 0; ip < nq; ++ip) qd[ip] = o[ud->plist ? ud->plist[ip] : ip];
     return 0;
 }
 
@@ -303,7 +318,8 @@ static int reinit_post_process_b(void* cvode_mem, N_Vector xB, double tnext) {
  * WRMS(xdot) < 1 with tolerances 100 x (rtol, atol) (solver.h ss_tol_factor).
  * On return x holds x_ss and sx (if given) dx_ss/dp (linear parameters). */
 static int preequilibrate(const amx_oracle_model* m, const amx_oracle_opts* o, ud_t* ud,
-                          const double* kpre, N_Vector x, N_Vector* sx, SUNContext ctx) {
+                          const double* kpre, int x0_given, N_Vector x, N_Vector* sx,
+                          SUNContext ctx) {
     const int nx = m->nx, np = m->np;
     int status = AMICI_SUCCESS;
     void* cv = NULL;
@@ -312,8 +328,14 @@ static int preequilibrate(const amx_oracle_model* m, const amx_oracle_opts* o, u
     int* plist = (int*)malloc(sizeof(int) * (size_t)(np > 0 ? np : 1));
     double* atols = (double*)malloc(sizeof(double) * (size_t)(np > 0 ? np : 1));
     for (int j = 0; j < np; ++j) { plist[j] = j; atols[j] = o->atol; }
+    const int* plist_saved = ud->plist;
+    ud->plist = NULL;
     double* xd = N_VGetArrayPointer(x);
-    for (int i = 0; i < nx; ++i) xd[i] = (m->x0k[i] >= 0) ? kpre[m->x0k[i]] : m->x0c[i];
+    /* x holds the initial state on entry; without ExpData::x0 the
+     * fixed-parameter-dependent species start from kpre (Model::initialize_states) */
+    if (!x0_given)
+        for (int i = 0; i < nx; ++i)
+            if (m->x0k[i] >= 0) xd[i] = kpre[m->x0k[i]];
     cv = CVodeCreate(CV_BDF, ctx);
     CHECK(CVodeInit(cv, cb_f, 0.0, x));
     CHECK(CVodeSStolerances(cv, o->rtol, o->atol));
@@ -346,6 +368,7 @@ static int preequilibrate(const amx_oracle_model* m, const amx_oracle_opts* o, u
         if (!(sqrt(acc / nx) < 1.0)) status = AMICI_NO_STEADY_STATE;
     }
 cleanup:
+    ud->plist = plist_saved;
     if (cv) CVodeFree(&cv);
     if (LS) SUNLinSolFree(LS);
     if (A) SUNMatDestroy(A);
@@ -354,11 +377,14 @@ cleanup:
 }
 
 int amx_oracle_simulate(const amx_oracle_model* m, const amx_oracle_opts* o,
-                        const double* theta, const double* kfix, const double* kpre, int nt,
+                        const double* theta, const double* kfix, const double* kpre,
+                        const double* x0, int nt,
                         const double* ts, const double* meas, const double* sigma,
                         double* x_out, double* y_out, double* sx_out, double* llh_out,
                         double* sllh_out, long* counters) {
-    const int nx = m->nx, np = m->np, ny = m->ny;
+    const int nx = m->nx, ny = m->ny, np_all = m->np;
+    /* sensitivities w.r.t. plist (edata.cpp:446-447, Model::set_parameter_list) */
+    const int np = o->plist ? o->nplist : m->np;
     const double t0 = 0.0;
     int status = AMICI_SUCCESS;
     SUNContext ctx = NULL;
@@ -370,12 +396,14 @@ int amx_oracle_simulate(const amx_oracle_model* m, const amx_oracle_opts* o,
     int which = 0;
     double llh = 0.0;
 
-    double* p = (double*)malloc(sizeof(double) * (size_t)np);
-    double* work = (double*)calloc((size_t)(3 * m->nr + nx * nx + nx * np + nx), sizeof(double));
+    double* p = (double*)malloc(sizeof(double) * (size_t)(np_all > 0 ? np_all : 1));
+    double* work = (double*)calloc((size_t)(3 * m->nr + nx * nx + nx * np_all + nx + np_all),
+                                   sizeof(double));
     double* dJydx = (double*)calloc((size_t)nt * nx, sizeof(double));
-    double* sllh = (double*)calloc((size_t)np, sizeof(double));
+    double* sllh = (double*)calloc((size_t)(np > 0 ? np : 1), sizeof(double));
     double* ybuf = (double*)malloc(sizeof(double) * (size_t)(ny > 0 ? ny : 1));
-    double* sx0 = (double*)calloc((size_t)nx * (np > 0 ? np : 1), sizeof(double)); /* [np][nx] */
+    /* [np_all][nx] initial state sensitivities of every parameter */
+    double* sx0 = (double*)calloc((size_t)nx * (np_all > 0 ? np_all : 1), sizeof(double));
     int* plist = (int*)malloc(sizeof(int) * (size_t)(np > 0 ? np : 1));
     double* atols = (double*)malloc(sizeof(double) * (size_t)(np > 0 ? np : 1));
     ud_t ud;
@@ -386,32 +414,39 @@ int amx_oracle_simulate(const amx_oracle_model* m, const amx_oracle_opts* o,
     ud.u = work + 2 * m->nr;
     ud.J = work + 3 * m->nr;
     ud.dfdp = ud.J + nx * nx;
+    ud.qfull = ud.dfdp + nx * np_all + nx;
+    ud.plist = plist;
     if (counters) memset(counters, 0, sizeof(long) * AMX_ORACLE_NCOUNTERS);
 
-    /* edata.cpp:438-557 + model.cpp:559: p = 10^theta for log10 scale */
-    for (int j = 0; j < np; ++j) { p[j] = pow(10.0, theta[j]); plist[j] = j; atols[j] = o->atol; }
+    /* edata.cpp:438-557 + model.cpp:559: p unscaled per parameter scale */
+    for (int j = 0; j < np_all; ++j) p[j] = unscale(theta[j], pscale_of(o, j));
+    for (int j = 0; j < np; ++j) { plist[j] = o->plist ? o->plist[j] : j; atols[j] = o->atol; }
 
     if (SUNContext_Create(SUN_COMM_NULL, &ctx) != 0) { status = AMICI_ERROR; goto cleanup; }
     x = N_VNew_Serial(nx, ctx);
+    {
+        /* initial state: ExpData::x0 if given (Model::set_initial_state),
+         * else the model's x0 (constants or fixed parameters) */
+        double* xd = N_VGetArrayPointer(x);
+        for (int i = 0; i < nx; ++i)
+            xd[i] = x0 ? x0[i] : ((m->x0k[i] >= 0) ? kfix[m->x0k[i]] : m->x0c[i]);
+    }
     if (kpre) {
         /* pre-equilibration, then the fixed-parameter-dependent initial states
          * are re-initialised from kfix (reinitialize_fixed_parameter_initial_states) */
-        N_Vector* sxp = (o->sensi >= 1 && np > 0) ? N_VCloneVectorArray(np, x) : NULL;
-        status = preequilibrate(m, o, &ud, kpre, x, sxp, ctx);
+        N_Vector* sxp = (o->sensi >= 1 && np > 0) ? N_VCloneVectorArray(np_all, x) : NULL;
+        status = preequilibrate(m, o, &ud, kpre, x0 != NULL, x, sxp, ctx);
         if (sxp) {
-            for (int j = 0; j < np; ++j) {
+            for (int j = 0; j < np_all; ++j) {
                 const double* s = N_VGetArrayPointer(sxp[j]);
                 for (int i = 0; i < nx; ++i) sx0[(size_t)j * nx + i] = (m->x0k[i] >= 0) ? 0.0 : s[i];
             }
-            N_VDestroyVectorArray(sxp, np);
+            N_VDestroyVectorArray(sxp, np_all);
         }
         if (status != AMICI_SUCCESS) goto cleanup;
         double* xd = N_VGetArrayPointer(x);
         for (int i = 0; i < nx; ++i)
             if (m->x0k[i] >= 0) xd[i] = kfix[m->x0k[i]];
-    } else {
-        double* xd = N_VGetArrayPointer(x);
-        for (int i = 0; i < nx; ++i) xd[i] = (m->x0k[i] >= 0) ? kfix[m->x0k[i]] : m->x0c[i];
     }
     cv = CVodeCreate(CV_BDF, ctx);
     CHECK(CVodeInit(cv, cb_f, t0, x));
@@ -427,7 +462,7 @@ int amx_oracle_simulate(const amx_oracle_model* m, const amx_oracle_opts* o,
     if (o->sensi == 1 && np > 0) {
         sx = N_VCloneVectorArray(np, x);
         for (int j = 0; j < np; ++j) /* sx0 = dx0/dp: 0, or from pre-equilibration */
-            memcpy(N_VGetArrayPointer(sx[j]), sx0 + (size_t)j * nx, sizeof(double) * (size_t)nx);
+            memcpy(N_VGetArrayPointer(sx[j]), sx0 + (size_t)plist[j] * nx, sizeof(double) * (size_t)nx);
         CHECK(CVodeSensInit1(cv, np, o->ism, cb_fS1, sx));
         CHECK(CVodeSetSensParams(cv, p, NULL, plist));
         CHECK(CVodeSensSStolerances(cv, o->rtol, atols));
@@ -585,21 +620,21 @@ int amx_oracle_simulate(const amx_oracle_model* m, const amx_oracle_opts* o,
         const double* bd = N_VGetArrayPointer(xB);
         for (int j = 0; j < np; ++j) {
             double xs = 0.0;
-            for (int i = 0; i < nx; ++i) xs += bd[i] * sx0[(size_t)j * nx + i];
+            for (int i = 0; i < nx; ++i) xs += bd[i] * sx0[(size_t)plist[j] * nx + i];
             sllh[j] -= xs + qd[j];
         }
     }
 
-    /* rdata.cpp:782-830: log10 chain rule on non-zero entries */
+    /* rdata.cpp:782-830: chain rule of each parameter's scale on non-zero entries */
     if (o->sensi >= 1) {
         for (int j = 0; j < np; ++j)
-            if (sllh[j] != 0.0) sllh[j] *= p[j] * log(10.0);
+            if (sllh[j] != 0.0) sllh[j] *= chain_coeff(p[plist[j]], pscale_of(o, plist[j]));
         if (sx_out && sx)
             for (int it = 0; it < nt; ++it)
                 for (int j = 0; j < np; ++j)
                     for (int i = 0; i < nx; ++i) {
                         double* v = sx_out + ((size_t)it * np + j) * nx + i;
-                        if (*v != 0.0) *v *= p[j] * log(10.0);
+                        if (*v != 0.0) *v *= chain_coeff(p[plist[j]], pscale_of(o, plist[j]));
                     }
     }
     if (llh_out) *llh_out = llh;
@@ -623,22 +658,24 @@ cleanup:
 
 int amx_oracle_simulate_batch(const amx_oracle_model* m, const amx_oracle_opts* o,
                               int nb, const double* theta, const double* kfix,
-                              const double* kpre, int nt, const double* ts, const double* meas,
+                              const double* kpre, const double* x0, int nt, const double* ts,
+                              const double* meas,
                               const double* sigma, double* x, double* y,
                               double* sx, double* llh, double* sllh,
                               long* counters, int* status, int nthreads) {
     const size_t nx = (size_t)m->nx, ny = (size_t)m->ny, np = (size_t)m->np, nk = (size_t)m->nk;
+    const size_t npl = o->plist ? (size_t)o->nplist : np;
     int nfail = 0;
 #pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1) reduction(+ : nfail)
     for (int b = 0; b < nb; ++b) {
         const size_t B = (size_t)b;
         int s = amx_oracle_simulate(
-            m, o, theta + B * np, kfix + B * nk, kpre ? kpre + B * nk : NULL, nt,
-            ts + B * (size_t)nt,
+            m, o, theta + B * np, kfix + B * nk, kpre ? kpre + B * nk : NULL,
+            x0 ? x0 + B * nx : NULL, nt, ts + B * (size_t)nt,
             meas ? meas + B * (size_t)nt * ny : NULL, sigma + B * (size_t)nt * ny,
             x ? x + B * (size_t)nt * nx : NULL, y ? y + B * (size_t)nt * ny : NULL,
-            sx ? sx + B * (size_t)nt * np * nx : NULL, llh ? llh + B : NULL,
-            sllh ? sllh + B * np : NULL,
+            sx ? sx + B * (size_t)nt * npl * nx : NULL, llh ? llh + B : NULL,
+            sllh ? sllh + B * npl : NULL,
             counters ? counters + B * AMX_ORACLE_NCOUNTERS : NULL);
         if (status) status[b] = s;
         if (s != 0) nfail++;
@@ -674,7 +711,9 @@ int amx_oracle_trace_forward(const amx_oracle_model* m, const amx_oracle_opts* o
     ud.u = work + 2 * m->nr;
     ud.J = work + 3 * m->nr;
     ud.dfdp = ud.J + nx * nx;
-    for (int j = 0; j < np; ++j) p[j] = pow(10.0, theta[j]);
+    ud.plist = NULL;
+    ud.qfull = NULL;
+    for (int j = 0; j < np; ++j) p[j] = unscale(theta[j], pscale_of(o, j));
     if (SUNContext_Create(SUN_COMM_NULL, &ctx) != 0) { status = AMICI_ERROR; goto cleanup; }
     x = N_VNew_Serial(nx, ctx);
     {

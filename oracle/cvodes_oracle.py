@@ -37,7 +37,8 @@ class _Opts(ctypes.Structure):
     _fields_ = [("rtol", ctypes.c_double), ("atol", ctypes.c_double),
                 ("quad_rtol", ctypes.c_double), ("quad_atol", ctypes.c_double),
                 ("maxsteps", ctypes.c_long), ("sensi", ctypes.c_int), ("ism", ctypes.c_int),
-                ("interp", ctypes.c_int), ("stldet", ctypes.c_int), ("t_preeq", ctypes.c_double)]
+                ("interp", ctypes.c_int), ("stldet", ctypes.c_int), ("t_preeq", ctypes.c_double),
+                ("nplist", ctypes.c_int), ("plist", ctypes.c_void_p), ("pscale", ctypes.c_void_p)]
 
 
 def build(reference: str = "/root/reference/ThirdParty/sundials") -> Path:
@@ -106,12 +107,15 @@ class OracleModel:
 def simulate(network, theta, kfix, ts, meas=None, sigma=None, *, rtol=1e-10, atol=1e-14,
              sensi=0, ism=CV_SIMULTANEOUS, interp=CV_POLYNOMIAL, stldet=1, maxsteps=100000,
              quad_rtol=1e-8, quad_atol=1e-12, nthreads=1, want_sx=False, kpre=None,
-             t_preeq=1e6):
-    """CVODES-oracle batch: theta [nb, np], kfix [nb, nk], ts [nt] or [nb, nt],
-    meas / sigma [nb, nt, ny].  sensi: 0 none, 1 forward, 2 adjoint.
+             t_preeq=1e6, plist=None, pscale=None, x0=None):
+    """CVODES-oracle batch: theta [nb, np] (on the scale pscale [np] of
+    ParameterScaling 0 none / 1 ln / 2 log10, default all log10), kfix [nb,
+    nk], ts [nt] or [nb, nt], meas / sigma [nb, nt, ny].  sensi: 0 none, 1
+    forward, 2 adjoint, with respect to the parameters plist (default all).
     kpre [nb, nk]: fixed parameters of a pre-equilibration to steady state
     (integration to t_preeq), after which the fixed-parameter-dependent
-    initial states are reset from kfix.
+    initial states are reset from kfix.  x0 [nb, nx]: initial states
+    replacing the model's.
     Returns dict of numpy arrays x, y, sx, llh, sllh, status, counters."""
     om = OracleModel(network)
     theta = np.ascontiguousarray(np.atleast_2d(theta), np.float64)
@@ -127,17 +131,22 @@ def simulate(network, theta, kfix, ts, meas=None, sigma=None, *, rtol=1e-10, ato
             if meas is not None else None)
     x = np.zeros((nb, nt, network.nx))
     y = np.zeros((nb, nt, network.ny))
-    sx = np.zeros((nb, nt, network.np, network.nx)) if (want_sx and sensi == 1) else None
+    pl = None if plist is None else np.ascontiguousarray(plist, np.int32)
+    ps = None if pscale is None else np.ascontiguousarray(pscale, np.int32)
+    npl = network.np if pl is None else len(pl)
+    x0a = None if x0 is None else np.ascontiguousarray(np.asarray(x0, np.float64).reshape(nb, network.nx))
+    sx = np.zeros((nb, nt, npl, network.nx)) if (want_sx and sensi == 1) else None
     llh = np.zeros(nb)
-    sllh = np.zeros((nb, network.np))
+    sllh = np.zeros((nb, npl))
     counters = np.zeros((nb, NCOUNTERS), np.int64)
     status = np.zeros(nb, np.int32)
-    opts = _Opts(rtol, atol, quad_rtol, quad_atol, maxsteps, sensi, ism, interp, stldet, t_preeq)
+    opts = _Opts(rtol, atol, quad_rtol, quad_atol, maxsteps, sensi, ism, interp, stldet, t_preeq,
+                 npl, _p(pl), _p(ps))
     if kpre is not None:
         kpre = np.ascontiguousarray(np.asarray(kpre, np.float64).reshape(nb, network.nk))
     lib().amx_oracle_simulate_batch(
         ctypes.byref(om.c), ctypes.byref(opts), ctypes.c_int(nb), _p(theta), _p(kfix), _p(kpre),
-        ctypes.c_int(nt), _p(ts), _p(meas), _p(sigma), _p(x), _p(y), _p(sx), _p(llh), _p(sllh),
+        _p(x0a), ctypes.c_int(nt), _p(ts), _p(meas), _p(sigma), _p(x), _p(y), _p(sx), _p(llh), _p(sllh),
         _p(counters), _p(status), ctypes.c_int(nthreads))
     return dict(x=x, y=y, sx=sx, llh=llh, sllh=sllh if sensi else None, status=status,
                 counters=counters)
@@ -152,7 +161,8 @@ def trace_forward(network, theta, kfix, ts, *, rtol=1e-8, atol=1e-12, stldet=1, 
     kfix = np.ascontiguousarray(np.asarray(kfix, np.float64).reshape(network.nk))
     ts = np.ascontiguousarray(ts, np.float64)
     rec = np.zeros((cap, 9))
-    opts = _Opts(rtol, atol, 1e-8, 1e-12, maxsteps, 0, CV_SIMULTANEOUS, CV_POLYNOMIAL, stldet, 0.0)
+    opts = _Opts(rtol, atol, 1e-8, 1e-12, maxsteps, 0, CV_SIMULTANEOUS, CV_POLYNOMIAL, stldet, 0.0,
+                 0, None, None)
     L = lib()
     n = L.amx_oracle_trace_forward(ctypes.byref(om.c), ctypes.byref(opts), _p(theta), _p(kfix),
                                    ctypes.c_int(len(ts)), _p(ts), _p(rec), ctypes.c_int(cap))
@@ -197,7 +207,8 @@ def stald_samples(network, theta, kfix, ts, *, rtol=1e-8, atol=1e-12, cap=50000)
     kfix = np.ascontiguousarray(np.asarray(kfix, np.float64).reshape(network.nk))
     ts = np.ascontiguousarray(ts, np.float64)
     out = np.zeros((cap, 16))
-    opts = _Opts(rtol, atol, 1e-8, 1e-12, 100000, 0, CV_SIMULTANEOUS, CV_POLYNOMIAL, 1, 0.0)
+    opts = _Opts(rtol, atol, 1e-8, 1e-12, 100000, 0, CV_SIMULTANEOUS, CV_POLYNOMIAL, 1, 0.0,
+                 0, None, None)
     n = _stald_lib().amx_oracle_stald_samples(
         ctypes.byref(om.c), ctypes.byref(opts), _p(theta), _p(kfix), ctypes.c_int(len(ts)),
         _p(ts), _p(out), ctypes.c_int(cap))

@@ -200,6 +200,34 @@ def n40_stald() -> None:
                         counters_off=off["counters"].astype(np.int32))
 
 
+def n40_plist_pscale() -> None:
+    """4 N40 conditions with a parameter list (5 parameters, not in order),
+    mixed parameter scales (parameter j on scale j mod 3: none, ln, log10;
+    theta holds the nominal log10 values converted to those scales) and
+    initial states 10 % off the model's (ExpData::x0): states, llh and the
+    forward and adjoint gradients at rtol 1e-10, plus the forward sx."""
+    g = np.load(OUT / "n40_conditions16.npz")
+    net = networks.network_n40()
+    nb = 4
+    p = 10.0 ** g["theta"][:nb]
+    pscale = np.arange(net.np) % 3
+    theta = np.where(pscale == 0, p, np.where(pscale == 1, np.log(p), np.log10(p)))
+    plist = np.array([117, 3, 64, 5, 40])
+    kfix = g["kfix"][:nb]
+    x0 = np.where(np.asarray(net.x0_fixed)[None] >= 0,
+                  np.take_along_axis(kfix, np.maximum(np.asarray(net.x0_fixed), 0)[None].repeat(nb, 0), 1),
+                  np.asarray(net.x0_const)[None])
+    x0 = x0 * (1.0 + 0.1 * np.sin(np.arange(net.nx)))[None]
+    args = (net, theta, kfix, g["ts"], g["meas"][:nb], g["sigma"][:nb])
+    kw = dict(plist=plist, pscale=pscale, x0=x0)
+    f = run(*args, 1, TIGHT, want_sx=True, **kw)
+    a = run(*args, 2, TIGHT, **kw)
+    np.savez_compressed(OUT / "n40_plist_pscale4.npz", theta=theta, kfix=kfix, ts=g["ts"],
+                        meas=g["meas"][:nb], sigma=g["sigma"][:nb], plist=plist, pscale=pscale,
+                        x0=x0, x=a["x"], y=a["y"], llh=a["llh"], sllh_fsa=f["sllh"],
+                        sllh_asa=a["sllh"], sx=f["sx"])
+
+
 def n200() -> None:
     """4 conditions of the 200-species network: states, observables, llh and
     the adjoint gradient at rtol 1e-10 / atol 1e-14, plus the rtol-1e-8
@@ -262,6 +290,8 @@ if __name__ == "__main__":
         n40_counters()
     elif sys.argv[1:] == ["n40_stald"]:
         n40_stald()
+    elif sys.argv[1:] == ["n40_plist_pscale"]:
+        n40_plist_pscale()
     elif sys.argv[1:] == ["n40_bench_accuracy"]:
         n40_bench_accuracy()
     elif sys.argv[1:] == ["robertson_mix"]:

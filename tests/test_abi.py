@@ -163,14 +163,16 @@ def test_teams_narrower_than_a_wavefront_are_refused():
         codegen.emit_model_traits(networks.network_n40(), 32)
 
 
-def test_abi_v5_struct_layout():
-    """amx_solver_opts ends with max_blocks, interp and stldet, amx_batch_out
-    with diag; 18 counters per instance (include/amx.h, ABI version 5)."""
-    assert _lib.AMX_ABI_VERSION == 5 and _lib.AMX_NSTATS == 18
+def test_abi_v6_struct_layout():
+    """amx_solver_opts ends with max_blocks, interp and stldet, amx_batch_in
+    with the parameter list, scales and initial states, amx_batch_out with
+    diag; 18 counters per instance (include/amx.h, ABI version 6)."""
+    assert _lib.AMX_ABI_VERSION == 6 and _lib.AMX_NSTATS == 18
+    assert [f[0] for f in _lib.BatchIn._fields_[-4:]] == ["nplist", "plist", "pscale", "x0"]
     assert [f[0] for f in _lib.SolverOpts._fields_[-3:]] == ["max_blocks", "interp", "stldet"]
     assert _lib.BatchOut._fields_[-1][0] == "diag"
     text = (ROOT / "include" / "amx.h").read_text()
-    assert "#define AMX_ABI_VERSION 5" in text and "#define AMX_NDIAG 9" in text
+    assert "#define AMX_ABI_VERSION 6" in text and "#define AMX_NDIAG 9" in text
     assert "#define AMX_NSTATS 18" in text
     assert "#define AMX_INTERP_HERMITE 1" in text and "#define AMX_INTERP_POLYNOMIAL 2" in text
     assert "#define AMX_STLDET_ON 1" in text and "#define AMX_STLDET_OFF 2" in text
@@ -215,3 +217,44 @@ def test_negative_max_blocks_rejected(lib):
     o = _lib.make_opts(max_blocks=-1)
     assert L.lib.amx_run_batch(ctypes.byref(o), ctypes.byref(ins), ctypes.byref(outs),
                                p, 8, None) == -22
+
+
+def test_parameter_list_length_rejected(lib):
+    """A parameter list longer than np is refused before any launch."""
+    net, L = lib
+    buf = (ctypes.c_double * 4)()
+    p = ctypes.cast(buf, ctypes.c_void_p).value
+    ins = _lib.BatchIn(1, 3, 0, p, p if net.nk else None, p, None, p, None, net.np + 1, p, None,
+                       None)
+    outs = _lib.BatchOut(None, None, None, p, p, p, None, None)
+    o = _lib.make_opts()
+    assert L.lib.amx_run_batch(ctypes.byref(o), ctypes.byref(ins), ctypes.byref(outs),
+                               p, 8, None) == -22
+
+
+def test_model_parameter_list_and_scale_api():
+    """Model.set_parameter_list / set_parameter_scale / set_initial_state
+    (include/amici/model.h:414-423, 954) validate their inputs; ExpData
+    carries plist, pscale and x0 (empty = the model's)."""
+    from amici_amd import sim
+    m = sim.Model(networks.linear_chain(3))
+    assert m.get_parameter_list() == list(range(m.np)) and m.nplist() == m.np
+    m.set_parameter_list([2, 0])
+    assert m.get_parameter_list() == [2, 0] and m.nplist() == 2
+    with pytest.raises(ValueError):
+        m.set_parameter_list([m.np])
+    m.set_parameter_scale("ln")
+    assert m.get_parameter_scale() == [_lib.PSCALE_LN] * m.np
+    m.set_parameter_scale(["none", 1, "log10", 2])
+    assert m.get_parameter_scale() == [0, 1, 2, 2]
+    with pytest.raises(ValueError):
+        m.set_parameter_scale([0, 1])
+    with pytest.raises(ValueError):
+        m.set_parameter_scale("log2")
+    x0 = np.arange(m.nx_rdata, dtype=float)
+    m.set_initial_state(x0)
+    assert np.array_equal(m.get_initial_state(), x0)
+    m.set_initial_state([])
+    assert np.array_equal(m.get_initial_state(), np.asarray(m.network.x0_const, float))
+    e = sim.ExpData(m)
+    assert list(e.plist) == [] and list(e.pscale) == [] and len(e.x0) == 0

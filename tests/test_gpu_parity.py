@@ -311,3 +311,59 @@ def test_mix_composition(gpu):
         _close(o["x"], g["x"], Y_RTOL, Y_FLOOR, f"{what} x")
         _check_llh(net, o, g)
         _close(o["sllh"], g["sllh_asa"], 1e-5, 1e-7, f"{what} sllh (adjoint)")
+
+
+@pytest.mark.parametrize("meth", [1, 2])
+def test_parameter_list_scales_and_initial_state(models, meth):
+    """Sensitivities with respect to a parameter list (5 parameters, not in
+    order), parameters on mixed scales (none / ln / log10) and initial states
+    given per condition (ExpData::plist / pscale / x0, applied by
+    ConditionContext::apply_condition, src/edata.cpp:438-490): states, llh,
+    sllh and sx against the oracle run with the same settings (fixture
+    n40_plist_pscale4), through simulate_batch and through run_simulations."""
+    from amici_amd import sim
+    g = _load("n40_plist_pscale4")
+    model = models["n40"]
+    kw = dict(plist=g["plist"], pscale=g["pscale"], x0=g["x0"])
+    out = _run_kw(model, g, meth, **kw)
+    assert (out["status"] == 0).all(), out["status"]
+    _close(out["x"], g["x"], Y_RTOL, Y_FLOOR, "x")
+    _check_llh(model.network, out, g)
+    assert out["sllh"].shape == (4, len(g["plist"]))
+    key = "sllh_fsa" if meth == 1 else "sllh_asa"
+    _close(out["sllh"], g[key], 1e-5, 1e-7, f"sllh ({meth})")
+    if meth == 1:
+        _close(out["sx"], g["sx"], 1e-5, 1e-7, "sx")
+    # the same through ExpData and run_simulations
+    sol = sim.Solver(**TIGHT)
+    sol.set_sensitivity_order(sim.SensitivityOrder.first)
+    sol.set_sensitivity_method(meth)
+    eds = []
+    for b in range(4):
+        e = sim.ExpData(model)
+        e.set_timepoints(g["ts"])
+        e.fixed_parameters = g["kfix"][b]
+        e.free_parameters = g["theta"][b]
+        e.set_measurements(g["meas"][b])
+        e.set_noise_scales(g["sigma"][b])
+        e.plist, e.pscale, e.x0 = list(g["plist"]), list(g["pscale"]), g["x0"][b]
+        eds.append(e)
+    rds = sim.run_simulations(model, sol, eds)
+    for b, rd in enumerate(rds):
+        assert rd.status == 0
+        np.testing.assert_array_equal(rd.sllh, out["sllh"][b])
+        np.testing.assert_array_equal(rd.x, out["x"][b])
+
+
+def _run_kw(model, g, meth, **kw):
+    import torch
+    from amici_amd import sim
+    sol = sim.Solver(**TIGHT)
+    if meth:
+        sol.set_sensitivity_order(sim.SensitivityOrder.first)
+        sol.set_sensitivity_method(meth)
+    out = sim.simulate_batch(model, sol, g["theta"], g["kfix"], g["ts"], g["meas"], g["sigma"],
+                             want_sx=(meth == 1), **kw)
+    torch.cuda.synchronize()
+    return {k: (v.cpu().numpy() if hasattr(v, "cpu") else v) for k, v in out.items()
+            if k != "_keepalive"}

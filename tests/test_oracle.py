@@ -304,3 +304,71 @@ def test_forward_trace_matches_counters():
         assert len(tr) == c[0] and tr[-1, 3] == c[0]
         assert tr[-1, 4] == c[1] and tr[-1, 5] == c[2]
         assert tr[-1, 0] == ts[-1] and np.all(np.diff(tr[:, 0]) > 0)
+
+
+def test_parameter_scales_by_finite_differences():
+    """Each parameter on its own scale (ParameterScaling none / ln / log10,
+    Model::set_parameter_scale): theta unscaled per parameter, and sllh the
+    gradient with respect to theta on that scale, by central differences of
+    llh in theta itself (linear chain, both sensitivity methods)."""
+    net = networks.linear_chain(3)
+    ts = np.linspace(0.0, 4.0, 9)
+    p = 10.0 ** net.theta_nominal
+    pscale = np.array([0, 1, 2, 0])[: net.np]
+    theta = np.where(pscale == 0, p, np.where(pscale == 1, np.log(p), np.log10(p)))
+    kw = dict(rtol=1e-12, atol=1e-16, pscale=pscale)
+    y = O.simulate(net, theta[None], np.zeros((1, 0)), ts, **kw)["y"][0]
+    ref = O.simulate(net, net.theta_nominal[None], np.zeros((1, 0)), ts, rtol=1e-12, atol=1e-16)
+    np.testing.assert_allclose(y, ref["y"][0], rtol=1e-12)
+    meas = y + 0.05
+    sig = np.full_like(meas, 0.1)
+    for sensi in (1, 2):
+        f = O.simulate(net, theta[None], np.zeros((1, 0)), ts, meas[None], sig[None], sensi=sensi,
+                       quad_rtol=1e-12, quad_atol=1e-16, **kw)
+        for j in range(net.np):
+            h = 1e-6 * max(abs(theta[j]), 1.0)
+            tp, tm = theta.copy(), theta.copy()
+            tp[j] += h
+            tm[j] -= h
+            lp = O.simulate(net, tp[None], np.zeros((1, 0)), ts, meas[None], sig[None], **kw)["llh"][0]
+            lm = O.simulate(net, tm[None], np.zeros((1, 0)), ts, meas[None], sig[None], **kw)["llh"][0]
+            assert f["sllh"][0][j] == pytest.approx((lp - lm) / (2 * h), rel=1e-5), (sensi, j)
+
+
+def test_parameter_list_selects_sensitivities():
+    """Sensitivities with respect to a parameter list (Model::set_parameter_list,
+    ExpData::plist; any order, a subset): sllh and sx are the columns of the
+    full run in the list's order, for both methods, up to the error-control
+    differences of integrating fewer sensitivities / quadratures."""
+    net = networks.network_n40()
+    g = np.load(GOLD / "n40_conditions16.npz")
+    th, kf, ts, me, sg = g["theta"][:2], g["kfix"][:2], g["ts"], g["meas"][:2], g["sigma"][:2]
+    kw = dict(rtol=1e-10, atol=1e-14, quad_rtol=1e-10, quad_atol=1e-14)
+    pl = [117, 3, 64, 5]
+    for sensi, key in ((1, "sllh_fsa"), (2, "sllh_asa")):
+        r = O.simulate(net, th, kf, ts, me, sg, sensi=sensi, plist=pl, want_sx=True, **kw)
+        assert r["sllh"].shape == (2, len(pl))
+        full = g[key][:2, pl]
+        assert np.abs(r["sllh"] - full).max() <= 1e-6 * np.abs(full).max()
+        if sensi == 1:
+            f = O.simulate(net, th, kf, ts, me, sg, sensi=1, want_sx=True, **kw)
+            np.testing.assert_allclose(r["sx"], f["sx"][:, :, pl], rtol=1e-6,
+                                       atol=1e-9 * np.abs(f["sx"]).max())
+
+
+def test_initial_state_override():
+    """ExpData::x0 replaces the model's initial state (Model::set_initial_state):
+    the linear chain from a given x0 follows expm(A t) x0, and x0 equal to the
+    model's own initial state reproduces the default run bit for bit."""
+    net = networks.linear_chain(3)
+    ts = np.array([0.0, 0.5, 2.0, 6.0])
+    x0 = np.array([0.3, 1.7, 0.2, 0.9, 0.4])[: net.nx]
+    r = O.simulate(net, net.theta_nominal[None], np.zeros((1, 0)), ts, rtol=1e-12, atol=1e-16,
+                   x0=x0[None])
+    A, _ = _linear_A(net, net.theta_nominal)
+    for it, t in enumerate(ts):
+        np.testing.assert_allclose(r["x"][0, it], expm(A * t) @ x0, rtol=1e-9, atol=1e-13)
+    a = O.simulate(net, net.theta_nominal[None], np.zeros((1, 0)), ts, rtol=1e-10, atol=1e-14)
+    b = O.simulate(net, net.theta_nominal[None], np.zeros((1, 0)), ts, rtol=1e-10, atol=1e-14,
+                   x0=net.x0_const[None])
+    assert np.array_equal(a["x"], b["x"])
