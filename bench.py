@@ -75,7 +75,7 @@ def flop_model(counters: np.ndarray, n: int, ops: dict, nt: int, ny: int) -> tup
 
 
 CONFIGS = {
-    "N40": dict(counters=("n40_counters1024.npz", "counters_asa"), cpu_sample=128,
+    "N40": dict(counters=("n40_counters1024.npz", "counters_asa"), cpu_sample=256,
                 plain_sample=8, traffic="n40_adjoint_hbm_traffic.json",
                 workload="N40 adjoint NLL+gradient: 40 species, 120 reactions, 1024 input "
                          "conditions, 25 times, 30 log-normal observables"),
