@@ -134,6 +134,13 @@ struct Launch {
                                    ? (BY_THREADS < BY_LDS ? BY_THREADS : BY_LDS) : 1;
   static constexpr int THREADS = TEAMS * M::T;
   static_assert(M::T % 64 == 0 && THREADS <= 256, "whole wavefronts per team, <= 256 threads");
+  // Register budget: models with at most 16 states are compiled for two
+  // waves per SIMD (at most 256 registers per lane), so batches larger than
+  // one instance per SIMD keep two resident per SIMD (4096 Robertson
+  // conditions with forward sensitivities: 110 ms per batch at two, 171 ms at
+  // one, 135 ms at four waves, where spills dominate); larger models keep the
+  // full 512 registers of one wave per SIMD.
+  static constexpr int WAVES_PER_EU = M::NX <= 16 ? 2 : 1;
 };
 
 // ---------------------------------------------------------------------------
@@ -341,7 +348,7 @@ struct FwdProblem {
   }
   __device__ __forceinline__ double sens_norm_slot(int slot) const {
     double m = 0.0;
-    for (int is = 0; is < ns; ++is) {
+    for (int is = 0; is < NS && is < ns; ++is) {
       const double e = (tm.lane < NX) ? S(slot, is) * S(EWT, is) : 0.0;
       m = fmax(m, sqrt(tm.sum(e * e) / NX));
     }
@@ -350,15 +357,15 @@ struct FwdProblem {
   __device__ __forceinline__ void sens_init() {
     if (tm.lane < NX)
       for (int sl = 0; sl < SENS_SLOTS; ++sl)
-        for (int is = 0; is < ns; ++is) S(sl, is) = 0.0;  // sx0 = dx0/dp = 0
+        for (int is = 0; is < NS && is < ns; ++is) S(sl, is) = 0.0;  // sx0 = dx0/dp = 0
   }
   __device__ __forceinline__ void sens_set_ewt() {
     if (tm.lane < NX)
-      for (int is = 0; is < ns; ++is) S(EWT, is) = 1.0 / (ssp_rtol * fabs(S(0, is)) + ssp_atol);
+      for (int is = 0; is < NS && is < ns; ++is) S(EWT, is) = 1.0 / (ssp_rtol * fabs(S(0, is)) + ssp_atol);
   }
   __device__ __forceinline__ void sens_predict(int q) {
     if (tm.lane >= NX) return;
-    for (int is = 0; is < ns; ++is) {
+    for (int is = 0; is < NS && is < ns; ++is) {
       double z[LMAX];
 #pragma unroll
       for (int j = 0; j < LMAX; ++j) z[j] = (j <= q) ? S(j, is) : 0.0;
@@ -374,7 +381,7 @@ struct FwdProblem {
   }
   __device__ __forceinline__ void sens_restore(int q) {
     if (tm.lane >= NX) return;
-    for (int is = 0; is < ns; ++is) {
+    for (int is = 0; is < NS && is < ns; ++is) {
       double z[LMAX];
 #pragma unroll
       for (int j = 0; j < LMAX; ++j) z[j] = (j <= q) ? S(j, is) : 0.0;
@@ -390,7 +397,7 @@ struct FwdProblem {
   }
   __device__ __forceinline__ void sens_rescale(int q, double eta) {
     if (tm.lane >= NX) return;
-    for (int is = 0; is < ns; ++is) {
+    for (int is = 0; is < NS && is < ns; ++is) {
       double c = eta;
 #pragma unroll
       for (int j = 1; j <= QMAX; ++j) {
@@ -401,7 +408,7 @@ struct FwdProblem {
   }
   __device__ __forceinline__ void sens_increase(int q, double A1, const double (&ll)[LMAX]) {
     if (tm.lane >= NX) return;
-    for (int is = 0; is < ns; ++is) {
+    for (int is = 0; is < NS && is < ns; ++is) {
       const double zL = A1 * S(QMAX, is);
 #pragma unroll
       for (int j = 2; j <= QMAX; ++j)
@@ -411,7 +418,7 @@ struct FwdProblem {
   }
   __device__ __forceinline__ void sens_decrease(int q, const double (&ll)[LMAX]) {
     if (tm.lane >= NX) return;
-    for (int is = 0; is < ns; ++is) {
+    for (int is = 0; is < NS && is < ns; ++is) {
       const double zq = S(q, is);
 #pragma unroll
       for (int j = 2; j < QMAX; ++j)
@@ -420,7 +427,7 @@ struct FwdProblem {
   }
   __device__ __forceinline__ void sens_complete(int q, const double (&l)[LMAX]) {
     if (tm.lane >= NX) return;
-    for (int is = 0; is < ns; ++is) {
+    for (int is = 0; is < NS && is < ns; ++is) {
       const double a = S(ACOR, is);
 #pragma unroll
       for (int j = 0; j <= QMAX; ++j)
@@ -429,12 +436,12 @@ struct FwdProblem {
   }
   __device__ __forceinline__ void sens_save_acor() {
     if (tm.lane >= NX) return;
-    for (int is = 0; is < ns; ++is) S(QMAX, is) = S(ACOR, is);
+    for (int is = 0; is < NS && is < ns; ++is) S(QMAX, is) = S(ACOR, is);
   }
   __device__ __forceinline__ double sens_norm_zn(int j) { return sens_norm_slot(j); }
   __device__ __forceinline__ double sens_norm_qp1(double cquot) {
     double m = 0.0;
-    for (int is = 0; is < ns; ++is) {
+    for (int is = 0; is < NS && is < ns; ++is) {
       const double e = (tm.lane < NX) ? (S(ACOR, is) - cquot * S(QMAX, is)) * S(EWT, is) : 0.0;
       m = fmax(m, sqrt(tm.sum(e * e) / NX));
     }
@@ -443,21 +450,21 @@ struct FwdProblem {
   __device__ __forceinline__ double sens_hub_inv() {
     double m = 0.0;
     if (tm.lane < NX)
-      for (int is = 0; is < ns; ++is)
+      for (int is = 0; is < NS && is < ns; ++is)
         m = fmax(m, fabs(S(1, is)) / (0.1 * fabs(S(0, is)) + 1.0 / S(EWT, is)));
     return tm.max(m);
   }
   // znS[1] = fS(t, y, znS[0]) (unscaled)
   __device__ __forceinline__ void sens_init_deriv(double, double y, double) {
     sens_prepare(y);
-    for (int is = 0; is < ns; ++is) {
+    for (int is = 0; is < NS && is < ns; ++is) {
       const double v = sens_f(tm.lane < NX ? S(0, is) : 0.0, is);
       if (tm.lane < NX) S(1, is) = v;
     }
   }
   __device__ __forceinline__ void sens_scale_deriv(double h) {
     if (tm.lane < NX)
-      for (int is = 0; is < ns; ++is) S(1, is) *= h;
+      for (int is = 0; is < NS && is < ns; ++is) S(1, is) *= h;
   }
   __device__ __forceinline__ void sens_restart_q1(double t, double y, double f, double h) {
     sens_init_deriv(t, y, f);
@@ -468,7 +475,7 @@ struct FwdProblem {
   __device__ __forceinline__ double sens_ydd_norm(double, double yy, double, double hg) {
     sens_prepare(yy);
     double m = 0.0;
-    for (int is = 0; is < ns; ++is) {
+    for (int is = 0; is < NS && is < ns; ++is) {
       const double ys = (tm.lane < NX) ? S(0, is) + hg * S(1, is) : 0.0;
       const double fs = sens_f(ys, is);
       const double e = (tm.lane < NX) ? (fs - S(1, is)) / hg * S(EWT, is) : 0.0;
@@ -485,7 +492,7 @@ struct FwdProblem {
     bool callSetup = false, jbad = false;
     for (;;) {
       if (tm.lane < NX)
-        for (int is = 0; is < ns; ++is) S(ACOR, is) = 0.0;
+        for (int is = 0; is < NS && is < ns; ++is) S(ACOR, is) = 0.0;
       if (callSetup) {
         if (jbad) b.convfail = FAIL_BAD_J;
         const int r = b.lsetup(b.convfail);
@@ -497,7 +504,7 @@ struct FwdProblem {
       for (;;) {
         b.nniS++;
         double Del = 0.0;
-        for (int is = 0; is < ns; ++is) {
+        for (int is = 0; is < NS && is < ns; ++is) {
           const double acS = (tm.lane < NX) ? S(ACOR, is) : 0.0;
           const double yS = (tm.lane < NX) ? S(0, is) + acS : 0.0;
           const double fS = sens_f(yS, is);
