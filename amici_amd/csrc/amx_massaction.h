@@ -56,6 +56,8 @@ struct BatchArgs {
   double* xss;             // adjoint: [nb][NX] pre-equilibrated steady state
   double* lu_ws;           // models with more than 64 states: [nb][NX][NX] Newton matrices
   int* counter;            // work-queue counter (zero before launch)
+  int* first_fail;         // failfast: lowest failed instance (INT_MAX before launch)
+  int failfast;            // skip instances after a failed one (run_simulations failfast)
   unsigned long long maxtime_ticks;  // per-instance wall-clock limit (0 = none)
   double* trace;           // optional step trace of one instance [cap][TRACE_REC]
   unsigned long long* prof;  // cycle profile [nb][2][PF_N] (diagnostic builds only)
@@ -1338,6 +1340,35 @@ __device__ __forceinline__ int next_instance(const Team<M::T>& tm, int* counter)
   return tm.bcast_i(v, 0);
 }
 
+// failfast (run_simulations' skipThrough, src/amici.cpp:232-260): once an
+// instance has failed, instances with a larger index that have not started
+// are not run (status AMICI_NOT_RUN, llh and sllh NaN); the caller reports
+// every instance after the first failed one as not run, so the results do
+// not depend on which of them had already started
+template <class M, class TM>
+__device__ __forceinline__ bool skip_failfast(const BatchArgs& a, const TM& tm, int inst) {
+  if (!a.failfast) return false;
+  int f = 0;
+  if (tm.lane == 0) f = __hip_atomic_load(a.first_fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  f = tm.bcast_i(f, 0);
+  if (inst <= f) return false;
+  const size_t I = (size_t)inst;
+  const int npl = a.npl(M::NP);
+  if (tm.lane == 0) {
+    a.status[inst] = AMX_NOT_RUN;
+    a.llh[inst] = NAN;
+    if (a.stats)
+      for (int k = 0; k < AMX_NSTATS; ++k) a.stats[I * AMX_NSTATS + k] = 0;
+  }
+  if (a.sllh)
+    for (int j = tm.lane; j < npl; j += M::T) a.sllh[I * npl + j] = NAN;
+  return true;
+}
+template <class TM>
+__device__ __forceinline__ void note_failure(const BatchArgs& a, const TM& tm, int inst) {
+  if (a.failfast && tm.lane == 0 && a.status[inst] < 0) atomicMin(a.first_fail, inst);
+}
+
 template <class M, int MODE>
 __device__ __forceinline__ void forward_kernel_body(BatchArgs a) {
   constexpr int T = M::T;
@@ -1353,7 +1384,9 @@ __device__ __forceinline__ void forward_kernel_body(BatchArgs a) {
   for (int guard = 0; guard <= a.nb; ++guard) {
     const int inst = next_instance<M>(tm, a.counter);
     if (inst >= a.nb) break;
+    if (skip_failfast<M>(a, tm, inst)) continue;
     simulate_forward<M, MODE>(inst, a, tm, s, lds);
+    note_failure(a, tm, inst);
   }
   }
 }
@@ -1369,7 +1402,9 @@ __device__ __forceinline__ void backward_kernel_body(BatchArgs a) {
   for (int guard = 0; guard <= a.nb; ++guard) {
     const int inst = next_instance<M>(tm, a.counter);
     if (inst >= a.nb) break;
+    if (skip_failfast<M>(a, tm, inst)) continue;
     simulate_backward<M, POLY>(inst, a, tm, s, lds);
+    note_failure(a, tm, inst);
   }
 }
 

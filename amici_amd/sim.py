@@ -482,7 +482,7 @@ def _ptr(t):
 
 def _prepare(model: Model, solver: Solver, theta, kfix, ts, meas, sigma, *, want_x, want_y,
              want_sx, want_stats, kfix_preeq, plist=None, pscale=None, x0=None,
-             copy_inputs=False) -> dict:
+             failfast=False, copy_inputs=False) -> dict:
     """Device inputs, output tensors and the C structs of one batch call.
     With ``copy_inputs`` every input is a fresh device copy (for a graph that
     keeps pointing at them).  plist / pscale default to the model's settings
@@ -530,6 +530,7 @@ def _prepare(model: Model, solver: Solver, theta, kfix, ts, meas, sigma, *, want
         raise ValueError("ts must be [nt] or [nb, nt]")
     meth = solver.effective_method()
     opts = solver.c_opts()
+    opts.failfast = 1 if failfast else 0
     f64 = dict(dtype=torch.float64, device=dev)
     out = {
         "x": torch.empty((nb, nt, model.nx_rdata), **f64) if want_x else None,
@@ -557,7 +558,8 @@ def _prepare(model: Model, solver: Solver, theta, kfix, ts, meas, sigma, *, want
 def simulate_batch(model: Model, solver: Solver, theta, kfix, ts, meas=None, sigma=None, *,
                    want_x: bool = True, want_y: bool = True, want_sx: bool = False,
                    want_stats: bool = True, sequential: bool = False, stream=None,
-                   kfix_preeq=None, plist=None, pscale=None, x0=None) -> dict:
+                   kfix_preeq=None, plist=None, pscale=None, x0=None,
+                   failfast: bool = False) -> dict:
     """Simulates ``nb`` instances in one batch; arrays may be numpy or torch.
 
     theta [nb, np] (on the parameter scales), kfix [nb, nk], ts [nt] or
@@ -567,13 +569,15 @@ def simulate_batch(model: Model, solver: Solver, theta, kfix, ts, meas=None, sig
     every instance to steady state with these fixed parameters before the
     simulation.  ``plist`` (parameter indices; sllh [nb, len(plist)], sx [nb,
     nt, len(plist), nx]), ``pscale`` (one ParameterScaling per parameter) and
-    ``x0`` [nb, nx] default to the model's settings.
+    ``x0`` [nb, nx] default to the model's settings.  With ``failfast`` the
+    kernels skip instances with a larger index than a failed one that have
+    not started yet (status AMICI_NOT_RUN).
     """
     import ctypes
 
     p = _prepare(model, solver, theta, kfix, ts, meas, sigma, want_x=want_x, want_y=want_y,
                  want_sx=want_sx, want_stats=want_stats, kfix_preeq=kfix_preeq, plist=plist,
-                 pscale=pscale, x0=x0)
+                 pscale=pscale, x0=x0, failfast=failfast)
     lib, dev, opts = model.lib, p["dev"], p["opts"]
     nbytes = lib.workspace_bytes(opts, 1 if sequential else p["nb"], p["nt"])
     st = stream if stream is not None else torch.cuda.current_stream(dev)
@@ -734,7 +738,7 @@ def run_simulations(model: Model, solver: Solver, edata_list: Sequence[ExpData],
           if has_x0 else None)
     out = simulate_batch(model, solver, theta, kfix, ts, meas, sigma, want_x="x" in fields,
                          want_sx=fsa, kfix_preeq=kpre, plist=list(plist), pscale=list(pscale),
-                         x0=x0)
+                         x0=x0, failfast=failfast)
     dev = out["llh"].device
     meas_t, sigma_t = out["_keepalive"][3], out["_keepalive"][4]
     sy_t = None

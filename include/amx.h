@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define AMX_ABI_VERSION 6
+#define AMX_ABI_VERSION 7
 #define AMX_NSTATS 18
 #define AMX_NDIAG 9
 /* bytes of the static workspace of amx_run_batch_static, per GPU */
@@ -70,6 +70,12 @@ typedef struct {
     int stldet;                /* AMX_STLDET_ON (AMICI's default, include/amici/solver.h:2037;
                                   0 means the same) or AMX_STLDET_OFF: BDF stability-limit
                                   detection in the forward and backward problems */
+    int failfast;              /* nonzero: once an instance fails, instances with a larger
+                                  index that have not started are skipped (status
+                                  AMICI_NOT_RUN, llh / sllh NaN), as run_simulations' failfast
+                                  (src/amici.cpp:232-260); which later instances had already
+                                  started depends on timing, so a caller reports every
+                                  instance after the first failed one as not run */
 } amx_solver_opts;
 
 typedef struct {

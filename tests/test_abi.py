@@ -163,16 +163,19 @@ def test_teams_narrower_than_a_wavefront_are_refused():
         codegen.emit_model_traits(networks.network_n40(), 32)
 
 
-def test_abi_v6_struct_layout():
-    """amx_solver_opts ends with max_blocks, interp and stldet, amx_batch_in
-    with the parameter list, scales and initial states, amx_batch_out with
-    diag; 18 counters per instance (include/amx.h, ABI version 6)."""
-    assert _lib.AMX_ABI_VERSION == 6 and _lib.AMX_NSTATS == 18
+def test_abi_v7_struct_layout():
+    """amx_solver_opts ends with max_blocks, interp, stldet and failfast,
+    amx_batch_in with the parameter list, scales and initial states,
+    amx_batch_out with diag; 18 counters per instance (include/amx.h, ABI
+    version 7)."""
+    assert _lib.AMX_ABI_VERSION == 7 and _lib.AMX_NSTATS == 18
     assert [f[0] for f in _lib.BatchIn._fields_[-4:]] == ["nplist", "plist", "pscale", "x0"]
-    assert [f[0] for f in _lib.SolverOpts._fields_[-3:]] == ["max_blocks", "interp", "stldet"]
+    assert [f[0] for f in _lib.SolverOpts._fields_[-4:]] == ["max_blocks", "interp", "stldet",
+                                                             "failfast"]
+    assert _lib.make_opts().failfast == 0 and _lib.make_opts(failfast=True).failfast == 1
     assert _lib.BatchOut._fields_[-1][0] == "diag"
     text = (ROOT / "include" / "amx.h").read_text()
-    assert "#define AMX_ABI_VERSION 6" in text and "#define AMX_NDIAG 9" in text
+    assert "#define AMX_ABI_VERSION 7" in text and "#define AMX_NDIAG 9" in text
     assert "#define AMX_NSTATS 18" in text
     assert "#define AMX_INTERP_HERMITE 1" in text and "#define AMX_INTERP_POLYNOMIAL 2" in text
     assert "#define AMX_STLDET_ON 1" in text and "#define AMX_STLDET_OFF 2" in text

@@ -288,3 +288,47 @@ def test_model_simulate(sim_models):
     one = model.simulate(solver=solver, edata=ed[1])
     assert one.sllh is None and one.status == 0
     np.testing.assert_array_equal(one.x, ref[1].x)
+
+
+@pytest.mark.parametrize("meth", [0, 2])
+def test_failfast_skips_on_device(sim_models, meth):
+    """With failfast the kernels skip the instances with a larger index than a
+    failed one that have not started yet (run_simulations' skipThrough,
+    src/amici.cpp:232-260).  One block of four teams works through 24
+    Robertson conditions; condition 0 fails at once (NaN parameters), so the
+    conditions its team and the others take from the queue afterwards never
+    run (no steps, status AMICI_NOT_RUN); the conditions that ran equal a run
+    without failfast, and run_simulations reports every condition after the
+    first failure as not run."""
+    import torch
+    from amici_amd import sim
+    model = sim_models["robertson"]
+    g = _load("robertson_batch64")
+    nb = 24
+    theta = g["theta"][:nb].copy()
+    theta[0] = np.nan
+    args = (theta, g["kfix"][:nb], g["ts"], g["meas"][:nb], g["sigma"][:nb])
+    sol = _solver(meth, max_blocks=1, rtol=1e-8, atol=1e-12)
+    ff = sim.simulate_batch(model, sol, *args, failfast=True)
+    ref = sim.simulate_batch(model, sol, *args)
+    torch.cuda.synchronize()
+    st, stats = ff["status"].cpu().numpy(), ff["stats"].cpu().numpy()
+    assert st[0] < 0 and ref["status"].cpu().numpy()[0] == st[0]
+    skipped = np.nonzero(st == sim.AMICI_NOT_RUN)[0]
+    assert len(skipped) >= nb - 8 and skipped.min() >= 1, st
+    assert (stats[skipped, 0] == 0).all() and np.isnan(ff["llh"].cpu().numpy()[skipped]).all()
+    ran = np.nonzero(st == 0)[0]
+    np.testing.assert_array_equal(ff["llh"].cpu().numpy()[ran], ref["llh"].cpu().numpy()[ran])
+    if meth:
+        np.testing.assert_array_equal(ff["sllh"].cpu().numpy()[ran], ref["sllh"].cpu().numpy()[ran])
+    # run_simulations: serial semantics on top
+    eds = []
+    for b in range(nb):
+        e = sim.ExpData(model)
+        e.set_timepoints(g["ts"])
+        e.free_parameters = theta[b]
+        e.set_measurements(g["meas"][b])
+        e.set_noise_scales(g["sigma"][b])
+        eds.append(e)
+    rds = sim.run_simulations(model, sol, eds, failfast=True)
+    assert rds[0].status < 0 and all(r.status == sim.AMICI_NOT_RUN for r in rds[1:])
