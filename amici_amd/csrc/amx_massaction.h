@@ -38,6 +38,7 @@ struct BatchArgs {
   int nplist;
   const int* pscale;       // [NP] ParameterScaling 0 none, 1 ln, 2 log10, or null (all log10)
   const double* x0;        // [nb][NX] initial states replacing the model's, or null
+  const double* sx0;       // [nb][npl][NX] their sensitivities w.r.t. the scaled parameters, or null
   int* status;             // [nb]
   int* stats;              // [nb][AMX_NSTATS] or null
   int* diag;               // [nb][nt][AMX_NDIAG] per-output-time counters or null
@@ -1052,7 +1053,18 @@ __device__ __forceinline__ void simulate_forward(int inst, const BatchArgs& a, c
     bdf.trace = a.trace;
     bdf.trace_cap = a.trace_cap / 2;  // second half: the backward problem's records
   }
-  if constexpr (MODE == 1) pb.sens_init();
+  if constexpr (MODE == 1) {
+    pb.sens_init();
+    // ExpData::sx0 (Model::set_initial_state_sensitivities, src/model.cpp:986-1023):
+    // given w.r.t. the scaled parameters, integrated divided by the chain-rule factor
+    if (a.sx0 && tm.lane < NX)
+      for (int is = 0; is < NS && is < pb.ns; ++is) {
+        const int pj = pb.pidx(is);
+        const int sc = a.scale(pj);
+        const double cf = chain_coeff(unscale_p(a.theta[I * NP + pj], sc), sc);
+        pb.S(0, is) = a.sx0[(I * npl + is) * NX + tm.lane] / cf;
+      }
+  }
   const double* ts = a.ts + (size_t)a.ts_stride * I;
 
   int status = pre_status;
@@ -1308,16 +1320,37 @@ __device__ __forceinline__ void simulate_backward(int inst, const BatchArgs& a, 
       }
     }
   }
-  // sllh = -(xB(t0).sx0 + qB) with sx0 = 0; chain rule of each parameter's
-  // scale on the non-zero entries
+  // caller-given initial state sensitivities (ExpData::sx0, w.r.t. the scaled
+  // parameters): -xB(t0) . sx0_j / (chain-rule factor of parameter j)
+  double gsx[EPQ];
+#pragma unroll
+  for (int e = 0; e < EPQ; ++e) gsx[e] = 0.0;
+  if (a.sx0) {
+    team_sync();
+    if (tm.lane < NX) s.v[tm.lane] = xB;
+    team_sync();
+#pragma unroll
+    for (int e = 0; e < EPQ; ++e) {
+      const int j = tm.lane + e * T;
+      if (j < npl) {
+        const double* sj = a.sx0 + (I * npl + j) * NX;
+        double d = 0.0;
+        for (int i = 0; i < NX; ++i) d = fma(s.v[i], sj[i], d);
+        gsx[e] = d;
+      }
+    }
+    team_sync();
+  }
+  // sllh = -(xB(t0).sx0 + qB); chain rule of each parameter's scale on the
+  // non-zero entries
 #pragma unroll
   for (int e = 0; e < EPQ; ++e) {
     const int j = tm.lane + e * T;
     if (j < npl) {
-      const double g = gss[e] - qB[e];
       const int pj = pb0.pidx(j);
       const int sc = a.scale(pj);
       const double cf = chain_coeff(unscale_p(a.theta[I * NP + pj], sc), sc);
+      const double g = gss[e] - qB[e] - gsx[e] / cf;
       a.sllh[I * npl + j] = (status == AMX_SUCCESS) ? (g != 0.0 ? g * cf : 0.0) : NAN;
     }
   }

@@ -67,6 +67,7 @@ class Model:
         self._plist = list(range(network.np))
         self._pscale = [_lib.PSCALE_LOG10] * network.np
         self._x0 = None
+        self._sx0 = None
 
     # dimensions
     @property
@@ -134,6 +135,24 @@ class Model:
         if x0.shape != (self.nx_rdata,):
             raise ValueError(f"expected {self.nx_rdata} initial states")
         self._x0 = x0.copy()
+
+    def set_initial_state_sensitivities(self, sx0) -> None:
+        """Initial state sensitivities with respect to the scaled parameters
+        of the parameter list (Model::set_initial_state_sensitivities,
+        src/model.cpp:986-1023; nplist x nx, AMICI's row-major order); an
+        empty sequence restores the model's own (zero)."""
+        sx0 = np.asarray(sx0, np.float64).ravel()
+        if sx0.size == 0:
+            self._sx0 = None
+            return
+        if sx0.size != self.nplist() * self.nx_rdata:
+            raise ValueError(f"expected {self.nplist()} x {self.nx_rdata} initial state sensitivities")
+        self._sx0 = sx0.reshape(self.nplist(), self.nx_rdata).copy()
+
+    def get_initial_state_sensitivities(self) -> np.ndarray:
+        if self._sx0 is not None:
+            return self._sx0.copy()
+        return np.zeros((self.nplist(), self.nx_rdata))
 
     def get_initial_state(self, kfix=None) -> np.ndarray:
         """The initial state used: the one set, else the model's (constants
@@ -338,8 +357,9 @@ class ExpData:
     parameters of a pre-equilibration (empty = none), and the optional
     overrides of the model's settings that ConditionContext::apply_condition
     applies (src/edata.cpp:438-490): free parameters (on their scale),
-    parameter list ``plist``, parameter scales ``pscale`` and initial states
-    ``x0`` (empty = the model's)."""
+    parameter list ``plist``, parameter scales ``pscale``, initial states
+    ``x0`` and their sensitivities ``sx0`` (nplist x nx, with respect to the
+    scaled parameters; empty = the model's)."""
 
     def __init__(self, source: "Model | ReturnData | None" = None, sigma_y=None,
                  sigma_z: float = 0.0, seed: int = -1):
@@ -357,7 +377,7 @@ class ExpData:
             self.fixed_parameters = np.zeros(0)
             self.free_parameters = np.zeros(0)
             self.fixed_parameters_pre_equilibration = np.zeros(0)
-            self.plist, self.pscale, self.x0 = [], [], np.zeros(0)
+            self.plist, self.pscale, self.x0, self.sx0 = [], [], np.zeros(0), np.zeros(0)
             nt = len(self.timepoints)
             sig = np.broadcast_to(np.asarray(1.0 if sigma_y is None else sigma_y, np.float64),
                                   (nt, self.ny)).copy()
@@ -374,7 +394,7 @@ class ExpData:
         self.fixed_parameters = model.get_fixed_parameters() if model else np.zeros(0)
         self.free_parameters = np.zeros(0)
         self.fixed_parameters_pre_equilibration = np.zeros(0)
-        self.plist, self.pscale, self.x0 = [], [], np.zeros(0)
+        self.plist, self.pscale, self.x0, self.sx0 = [], [], np.zeros(0), np.zeros(0)
         self._meas = np.full((len(self.timepoints), self.ny), np.nan)
         self._sigma = np.ones((len(self.timepoints), self.ny))
         self.id = ""
@@ -482,7 +502,7 @@ def _ptr(t):
 
 def _prepare(model: Model, solver: Solver, theta, kfix, ts, meas, sigma, *, want_x, want_y,
              want_sx, want_stats, kfix_preeq, plist=None, pscale=None, x0=None,
-             failfast=False, copy_inputs=False) -> dict:
+             sx0=None, failfast=False, copy_inputs=False) -> dict:
     """Device inputs, output tensors and the C structs of one batch call.
     With ``copy_inputs`` every input is a fresh device copy (for a graph that
     keeps pointing at them).  plist / pscale default to the model's settings
@@ -496,6 +516,8 @@ def _prepare(model: Model, solver: Solver, theta, kfix, ts, meas, sigma, *, want
     npl = len(plist)
     if x0 is None and model._x0 is not None:
         x0 = np.broadcast_to(model._x0, (np.shape(theta)[0], model.nx_rdata))
+    if sx0 is None and model._sx0 is not None and model._sx0.shape[0] == npl:
+        sx0 = np.broadcast_to(model._sx0, (np.shape(theta)[0], npl, model.nx_rdata))
 
     def dev_t(a):
         t = _as_dev(a, dev)
@@ -515,6 +537,9 @@ def _prepare(model: Model, solver: Solver, theta, kfix, ts, meas, sigma, *, want
     x0_t = dev_t(x0) if x0 is not None else None
     if x0_t is not None and x0_t.shape != (nb, model.nx_rdata):
         raise ValueError("x0 must be [nb, nx]")
+    sx0_t = dev_t(sx0) if sx0 is not None else None
+    if sx0_t is not None and sx0_t.shape != (nb, npl, model.nx_rdata):
+        raise ValueError("sx0 must be [nb, nplist, nx]")
     # the identity list and all-log10 scales are the kernels' defaults (NULL)
     plist_t = (None if plist == list(range(model.np))
                else torch.tensor(plist, dtype=torch.int32, device=dev))
@@ -547,18 +572,18 @@ def _prepare(model: Model, solver: Solver, theta, kfix, ts, meas, sigma, *, want
     }
     bin_ = _lib.BatchIn(nb, nt, ts_stride, _ptr(theta_t), _ptr(kfix_t) if model.nk else None,
                         _ptr(ts_t), _ptr(meas_t), _ptr(sigma_t), _ptr(kpre_t),
-                        npl, _ptr(plist_t), _ptr(pscale_t), _ptr(x0_t))
+                        npl, _ptr(plist_t), _ptr(pscale_t), _ptr(x0_t), _ptr(sx0_t))
     bout = _lib.BatchOut(*[_ptr(out[k]) for k in ("x", "y", "sx", "llh", "sllh", "status", "stats",
                                                   "diag")])
     inputs = dict(theta=theta_t, kfix=kfix_t, ts=ts_t, meas=meas_t, sigma=sigma_t,
-                  kfix_preeq=kpre_t, plist=plist_t, pscale=pscale_t, x0=x0_t)
+                  kfix_preeq=kpre_t, plist=plist_t, pscale=pscale_t, x0=x0_t, sx0=sx0_t)
     return dict(dev=dev, nb=nb, nt=nt, opts=opts, bin=bin_, bout=bout, out=out, inputs=inputs)
 
 
 def simulate_batch(model: Model, solver: Solver, theta, kfix, ts, meas=None, sigma=None, *,
                    want_x: bool = True, want_y: bool = True, want_sx: bool = False,
                    want_stats: bool = True, sequential: bool = False, stream=None,
-                   kfix_preeq=None, plist=None, pscale=None, x0=None,
+                   kfix_preeq=None, plist=None, pscale=None, x0=None, sx0=None,
                    failfast: bool = False) -> dict:
     """Simulates ``nb`` instances in one batch; arrays may be numpy or torch.
 
@@ -569,7 +594,8 @@ def simulate_batch(model: Model, solver: Solver, theta, kfix, ts, meas=None, sig
     every instance to steady state with these fixed parameters before the
     simulation.  ``plist`` (parameter indices; sllh [nb, len(plist)], sx [nb,
     nt, len(plist), nx]), ``pscale`` (one ParameterScaling per parameter) and
-    ``x0`` [nb, nx] default to the model's settings.  With ``failfast`` the
+    ``x0`` [nb, nx] and ``sx0`` [nb, nplist, nx] (with respect to the scaled
+    parameters) default to the model's settings.  With ``failfast`` the
     kernels skip instances with a larger index than a failed one that have
     not started yet (status AMICI_NOT_RUN).
     """
@@ -577,7 +603,7 @@ def simulate_batch(model: Model, solver: Solver, theta, kfix, ts, meas=None, sig
 
     p = _prepare(model, solver, theta, kfix, ts, meas, sigma, want_x=want_x, want_y=want_y,
                  want_sx=want_sx, want_stats=want_stats, kfix_preeq=kfix_preeq, plist=plist,
-                 pscale=pscale, x0=x0, failfast=failfast)
+                 pscale=pscale, x0=x0, sx0=sx0, failfast=failfast)
     lib, dev, opts = model.lib, p["dev"], p["opts"]
     nbytes = lib.workspace_bytes(opts, 1 if sequential else p["nb"], p["nt"])
     st = stream if stream is not None else torch.cuda.current_stream(dev)
@@ -604,12 +630,13 @@ class BatchGraph:
 
     def __init__(self, model: Model, solver: Solver, theta, kfix, ts, meas=None, sigma=None, *,
                  want_x: bool = True, want_y: bool = True, want_sx: bool = False,
-                 want_stats: bool = True, kfix_preeq=None, plist=None, pscale=None, x0=None):
+                 want_stats: bool = True, kfix_preeq=None, plist=None, pscale=None, x0=None,
+                 sx0=None):
         import ctypes
 
         p = _prepare(model, solver, theta, kfix, ts, meas, sigma, want_x=want_x, want_y=want_y,
                      want_sx=want_sx, want_stats=want_stats, kfix_preeq=kfix_preeq,
-                     plist=plist, pscale=pscale, x0=x0, copy_inputs=True)
+                     plist=plist, pscale=pscale, x0=x0, sx0=sx0, copy_inputs=True)
         self.model = model
         self.inputs = p["inputs"]
         self.out = p["out"]
@@ -702,7 +729,7 @@ def run_simulations(model: Model, solver: Solver, edata_list: Sequence[ExpData],
         pl = tuple(int(i) for i in e.plist) if len(e.plist) else tuple(model.get_parameter_list())
         ps = (tuple(_pscale_list(e.pscale, model.np)) if len(e.pscale)
               else tuple(model.get_parameter_scale()))
-        return (len(e.timepoints), pl, ps, len(e.x0) > 0)
+        return (len(e.timepoints), pl, ps, len(e.x0) > 0, len(e.sx0) > 0)
 
     keys = [key(e) for e in edata_list]
     if len(set(keys)) != 1:
@@ -713,7 +740,7 @@ def run_simulations(model: Model, solver: Solver, edata_list: Sequence[ExpData],
             for i, r in zip(idx, sub):
                 res[i] = r
         return _apply_failfast(res, failfast)
-    _, plist, pscale, has_x0 = keys[0]
+    _, plist, pscale, has_x0, has_sx0 = keys[0]
     theta = np.stack([e.free_parameters if len(e.free_parameters) else model.get_free_parameters()
                       for e in edata_list])
     kfix = np.stack([np.asarray(e.fixed_parameters, np.float64) if len(e.fixed_parameters)
@@ -736,9 +763,11 @@ def run_simulations(model: Model, solver: Solver, edata_list: Sequence[ExpData],
                                     sensi=meth != _lib.SENSI_NONE)
     x0 = (np.stack([np.asarray(e.x0, np.float64).reshape(model.nx_rdata) for e in edata_list])
           if has_x0 else None)
+    sx0 = (np.stack([np.asarray(e.sx0, np.float64).reshape(len(plist), model.nx_rdata)
+                     for e in edata_list]) if has_sx0 else None)
     out = simulate_batch(model, solver, theta, kfix, ts, meas, sigma, want_x="x" in fields,
                          want_sx=fsa, kfix_preeq=kpre, plist=list(plist), pscale=list(pscale),
-                         x0=x0, failfast=failfast)
+                         x0=x0, sx0=sx0, failfast=failfast)
     dev = out["llh"].device
     meas_t, sigma_t = out["_keepalive"][3], out["_keepalive"][4]
     sy_t = None

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define AMX_ABI_VERSION 7
+#define AMX_ABI_VERSION 8
 #define AMX_NSTATS 18
 #define AMX_NDIAG 9
 /* bytes of the static workspace of amx_run_batch_static, per GPU */
@@ -102,11 +102,15 @@ typedef struct {
      *   AMICI_ILL_INPUT (-22).
      * x0: [nb][nx] initial states replacing the model's (ExpData::x0,
      *   Model::set_initial_state; [the model's]); a pre-equilibration starts
-     *   from it too.                                                          */
+     *   from it too.
+     * sx0: [nb][nplist][nx] their sensitivities with respect to the scaled
+     *   parameters (ExpData::sx0, Model::set_initial_state_sensitivities;
+     *   [0]); not together with kfix_preeq (-22).                            */
     int nplist;
     const int* plist;
     const int* pscale;
     const double* x0;
+    const double* sx0;
 } amx_batch_in;
 
 typedef struct {

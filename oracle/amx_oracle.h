@@ -63,7 +63,9 @@ typedef struct {
 
 /* One condition.  theta: [np] parameters on their scale (pscale); kfix: [nk];
  * kpre: [nk] fixed parameters of a pre-equilibration (NULL = none); x0: [nx]
- * initial state replacing the model's (ExpData::x0, NULL = none); ts: [nt];
+ * initial state replacing the model's (ExpData::x0, NULL = none); sx0:
+ * [nplist*nx] initial state sensitivities with respect to the scaled
+ * parameters (ExpData::sx0, NULL = none; not with kpre); ts: [nt];
  * meas/sigma: [nt*ny] (NaN measurement = missing).
  * Outputs (may be NULL): x [nt*nx], y [nt*ny], sx [nt*nplist*nx] (FSA only),
  * llh [1], sllh [nplist] (w.r.t. the scaled parameters), counters
@@ -71,7 +73,7 @@ typedef struct {
  * Returns AMICI status (0 success, <0 failure as in include/amici/defines.h). */
 int amx_oracle_simulate(const amx_oracle_model* m, const amx_oracle_opts* o,
                         const double* theta, const double* kfix, const double* kpre,
-                        const double* x0, int nt,
+                        const double* x0, const double* sx0, int nt,
                         const double* ts, const double* meas, const double* sigma,
                         double* x, double* y, double* sx, double* llh,
                         double* sllh, long* counters);
@@ -80,7 +82,8 @@ int amx_oracle_simulate(const amx_oracle_model* m, const amx_oracle_opts* o,
  * Per-condition arrays are strided by their single-condition size. */
 int amx_oracle_simulate_batch(const amx_oracle_model* m, const amx_oracle_opts* o,
                               int nb, const double* theta, const double* kfix,
-                              const double* kpre, const double* x0, int nt, const double* ts,
+                              const double* kpre, const double* x0, const double* sx0,
+                              int nt, const double* ts,
                               const double* meas,
                               const double* sigma, double* x, double* y,
                               double* sx, double* llh, double* sllh,

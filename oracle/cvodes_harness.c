@@ -378,7 +378,7 @@ cleanup:
 
 int amx_oracle_simulate(const amx_oracle_model* m, const amx_oracle_opts* o,
                         const double* theta, const double* kfix, const double* kpre,
-                        const double* x0, int nt,
+                        const double* x0, const double* sx0_in, int nt,
                         const double* ts, const double* meas, const double* sigma,
                         double* x_out, double* y_out, double* sx_out, double* llh_out,
                         double* sllh_out, long* counters) {
@@ -447,6 +447,14 @@ int amx_oracle_simulate(const amx_oracle_model* m, const amx_oracle_opts* o,
         double* xd = N_VGetArrayPointer(x);
         for (int i = 0; i < nx; ++i)
             if (m->x0k[i] >= 0) xd[i] = kfix[m->x0k[i]];
+    } else if (sx0_in) {
+        /* ExpData::sx0 (Model::set_initial_state_sensitivities, model.cpp:986-1023):
+         * given w.r.t. the scaled parameters, stored divided by the chain-rule factor */
+        for (int j = 0; j < np; ++j) {
+            const double cf = chain_coeff(p[plist[j]], pscale_of(o, plist[j]));
+            for (int i = 0; i < nx; ++i)
+                sx0[(size_t)plist[j] * nx + i] = sx0_in[(size_t)j * nx + i] / cf;
+        }
     }
     cv = CVodeCreate(CV_BDF, ctx);
     CHECK(CVodeInit(cv, cb_f, t0, x));
@@ -658,7 +666,8 @@ cleanup:
 
 int amx_oracle_simulate_batch(const amx_oracle_model* m, const amx_oracle_opts* o,
                               int nb, const double* theta, const double* kfix,
-                              const double* kpre, const double* x0, int nt, const double* ts,
+                              const double* kpre, const double* x0, const double* sx0,
+                              int nt, const double* ts,
                               const double* meas,
                               const double* sigma, double* x, double* y,
                               double* sx, double* llh, double* sllh,
@@ -671,7 +680,7 @@ int amx_oracle_simulate_batch(const amx_oracle_model* m, const amx_oracle_opts* 
         const size_t B = (size_t)b;
         int s = amx_oracle_simulate(
             m, o, theta + B * np, kfix + B * nk, kpre ? kpre + B * nk : NULL,
-            x0 ? x0 + B * nx : NULL, nt, ts + B * (size_t)nt,
+            x0 ? x0 + B * nx : NULL, sx0 ? sx0 + B * npl * nx : NULL, nt, ts + B * (size_t)nt,
             meas ? meas + B * (size_t)nt * ny : NULL, sigma + B * (size_t)nt * ny,
             x ? x + B * (size_t)nt * nx : NULL, y ? y + B * (size_t)nt * ny : NULL,
             sx ? sx + B * (size_t)nt * npl * nx : NULL, llh ? llh + B : NULL,

@@ -107,7 +107,7 @@ class OracleModel:
 def simulate(network, theta, kfix, ts, meas=None, sigma=None, *, rtol=1e-10, atol=1e-14,
              sensi=0, ism=CV_SIMULTANEOUS, interp=CV_POLYNOMIAL, stldet=1, maxsteps=100000,
              quad_rtol=1e-8, quad_atol=1e-12, nthreads=1, want_sx=False, kpre=None,
-             t_preeq=1e6, plist=None, pscale=None, x0=None):
+             t_preeq=1e6, plist=None, pscale=None, x0=None, sx0=None):
     """CVODES-oracle batch: theta [nb, np] (on the scale pscale [np] of
     ParameterScaling 0 none / 1 ln / 2 log10, default all log10), kfix [nb,
     nk], ts [nt] or [nb, nt], meas / sigma [nb, nt, ny].  sensi: 0 none, 1
@@ -115,7 +115,8 @@ def simulate(network, theta, kfix, ts, meas=None, sigma=None, *, rtol=1e-10, ato
     kpre [nb, nk]: fixed parameters of a pre-equilibration to steady state
     (integration to t_preeq), after which the fixed-parameter-dependent
     initial states are reset from kfix.  x0 [nb, nx]: initial states
-    replacing the model's.
+    replacing the model's; sx0 [nb, nplist, nx]: their sensitivities with
+    respect to the scaled parameters (without kpre).
     Returns dict of numpy arrays x, y, sx, llh, sllh, status, counters."""
     om = OracleModel(network)
     theta = np.ascontiguousarray(np.atleast_2d(theta), np.float64)
@@ -135,6 +136,8 @@ def simulate(network, theta, kfix, ts, meas=None, sigma=None, *, rtol=1e-10, ato
     ps = None if pscale is None else np.ascontiguousarray(pscale, np.int32)
     npl = network.np if pl is None else len(pl)
     x0a = None if x0 is None else np.ascontiguousarray(np.asarray(x0, np.float64).reshape(nb, network.nx))
+    sx0a = (None if sx0 is None
+            else np.ascontiguousarray(np.asarray(sx0, np.float64).reshape(nb, npl, network.nx)))
     sx = np.zeros((nb, nt, npl, network.nx)) if (want_sx and sensi == 1) else None
     llh = np.zeros(nb)
     sllh = np.zeros((nb, npl))
@@ -146,7 +149,7 @@ def simulate(network, theta, kfix, ts, meas=None, sigma=None, *, rtol=1e-10, ato
         kpre = np.ascontiguousarray(np.asarray(kpre, np.float64).reshape(nb, network.nk))
     lib().amx_oracle_simulate_batch(
         ctypes.byref(om.c), ctypes.byref(opts), ctypes.c_int(nb), _p(theta), _p(kfix), _p(kpre),
-        _p(x0a), ctypes.c_int(nt), _p(ts), _p(meas), _p(sigma), _p(x), _p(y), _p(sx), _p(llh), _p(sllh),
+        _p(x0a), _p(sx0a), ctypes.c_int(nt), _p(ts), _p(meas), _p(sigma), _p(x), _p(y), _p(sx), _p(llh), _p(sllh),
         _p(counters), _p(status), ctypes.c_int(nthreads))
     return dict(x=x, y=y, sx=sx, llh=llh, sllh=sllh if sensi else None, status=status,
                 counters=counters)

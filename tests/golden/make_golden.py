@@ -228,6 +228,25 @@ def n40_plist_pscale() -> None:
                         sllh_asa=a["sllh"], sx=f["sx"])
 
 
+def robertson_sx0() -> None:
+    """8 Robertson conditions with caller-given initial state sensitivities
+    (ExpData::sx0, with respect to the log10 parameters) for a parameter list
+    [2, 0]: states, llh, the forward and adjoint gradients and sx at rtol
+    1e-10."""
+    g = np.load(OUT / "robertson_batch64.npz")
+    net = networks.robertson()
+    nb = 8
+    plist = np.array([2, 0])
+    sx0 = np.random.default_rng(11).normal(scale=0.1, size=(nb, len(plist), net.nx))
+    args = (net, g["theta"][:nb], g["kfix"][:nb], g["ts"], g["meas"][:nb], g["sigma"][:nb])
+    f = run(*args, 1, TIGHT, want_sx=True, plist=plist, sx0=sx0)
+    a = run(*args, 2, TIGHT, plist=plist, sx0=sx0)
+    np.savez_compressed(OUT / "robertson_sx0.npz", theta=g["theta"][:nb], kfix=g["kfix"][:nb],
+                        ts=g["ts"], meas=g["meas"][:nb], sigma=g["sigma"][:nb], plist=plist,
+                        sx0=sx0, x=a["x"], y=a["y"], llh=a["llh"], sllh_fsa=f["sllh"],
+                        sllh_asa=a["sllh"], sx=f["sx"])
+
+
 def n200() -> None:
     """4 conditions of the 200-species network: states, observables, llh and
     the adjoint gradient at rtol 1e-10 / atol 1e-14, plus the rtol-1e-8
@@ -292,6 +311,8 @@ if __name__ == "__main__":
         n40_stald()
     elif sys.argv[1:] == ["n40_plist_pscale"]:
         n40_plist_pscale()
+    elif sys.argv[1:] == ["robertson_sx0"]:
+        robertson_sx0()
     elif sys.argv[1:] == ["n40_bench_accuracy"]:
         n40_bench_accuracy()
     elif sys.argv[1:] == ["robertson_mix"]:

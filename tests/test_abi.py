@@ -163,19 +163,19 @@ def test_teams_narrower_than_a_wavefront_are_refused():
         codegen.emit_model_traits(networks.network_n40(), 32)
 
 
-def test_abi_v7_struct_layout():
+def test_abi_v8_struct_layout():
     """amx_solver_opts ends with max_blocks, interp, stldet and failfast,
-    amx_batch_in with the parameter list, scales and initial states,
-    amx_batch_out with diag; 18 counters per instance (include/amx.h, ABI
-    version 7)."""
-    assert _lib.AMX_ABI_VERSION == 7 and _lib.AMX_NSTATS == 18
-    assert [f[0] for f in _lib.BatchIn._fields_[-4:]] == ["nplist", "plist", "pscale", "x0"]
+    amx_batch_in with the parameter list, scales, initial states and their
+    sensitivities, amx_batch_out with diag; 18 counters per instance
+    (include/amx.h, ABI version 8)."""
+    assert _lib.AMX_ABI_VERSION == 8 and _lib.AMX_NSTATS == 18
+    assert [f[0] for f in _lib.BatchIn._fields_[-5:]] == ["nplist", "plist", "pscale", "x0", "sx0"]
     assert [f[0] for f in _lib.SolverOpts._fields_[-4:]] == ["max_blocks", "interp", "stldet",
                                                              "failfast"]
     assert _lib.make_opts().failfast == 0 and _lib.make_opts(failfast=True).failfast == 1
     assert _lib.BatchOut._fields_[-1][0] == "diag"
     text = (ROOT / "include" / "amx.h").read_text()
-    assert "#define AMX_ABI_VERSION 7" in text and "#define AMX_NDIAG 9" in text
+    assert "#define AMX_ABI_VERSION 8" in text and "#define AMX_NDIAG 9" in text
     assert "#define AMX_NSTATS 18" in text
     assert "#define AMX_INTERP_HERMITE 1" in text and "#define AMX_INTERP_POLYNOMIAL 2" in text
     assert "#define AMX_STLDET_ON 1" in text and "#define AMX_STLDET_OFF 2" in text
@@ -261,3 +261,18 @@ def test_model_parameter_list_and_scale_api():
     assert np.array_equal(m.get_initial_state(), np.asarray(m.network.x0_const, float))
     e = sim.ExpData(m)
     assert list(e.plist) == [] and list(e.pscale) == [] and len(e.x0) == 0
+
+
+def test_sx0_with_preequilibration_rejected(lib):
+    """Caller-given initial state sensitivities together with a
+    pre-equilibration are refused before any launch."""
+    net, L = lib
+    if not net.nk:
+        return
+    buf = (ctypes.c_double * 4)()
+    p = ctypes.cast(buf, ctypes.c_void_p).value
+    ins = _lib.BatchIn(1, 3, 0, p, p, p, None, p, p, 0, None, None, None, p)
+    outs = _lib.BatchOut(None, None, None, p, p, p, None, None)
+    o = _lib.make_opts(sensi_meth=_lib.SENSI_FORWARD)
+    assert L.lib.amx_run_batch(ctypes.byref(o), ctypes.byref(ins), ctypes.byref(outs),
+                               p, 8, None) == -22

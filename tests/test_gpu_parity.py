@@ -367,3 +367,29 @@ def _run_kw(model, g, meth, **kw):
     torch.cuda.synchronize()
     return {k: (v.cpu().numpy() if hasattr(v, "cpu") else v) for k, v in out.items()
             if k != "_keepalive"}
+
+
+@pytest.mark.parametrize("meth", [1, 2])
+def test_initial_state_sensitivities(models, meth):
+    """Caller-given initial state sensitivities (ExpData::sx0, with respect to
+    the scaled parameters, for a parameter list [2, 0]) on 8 Robertson
+    conditions: sllh and sx against the oracle (fixture robertson_sx0), and
+    through Model.set_initial_state_sensitivities."""
+    from amici_amd import sim
+    g = _load("robertson_sx0")
+    model = models["robertson"]
+    out = _run_kw(model, g, meth, plist=g["plist"], sx0=g["sx0"])
+    assert (out["status"] == 0).all(), out["status"]
+    key = "sllh_fsa" if meth == 1 else "sllh_asa"
+    _close(out["sllh"], g[key], 1e-5, 1e-7, f"sllh ({meth})")
+    if meth == 1:
+        _close(out["sx"], g["sx"], 1e-5, 1e-7, "sx")
+    model.set_parameter_list(list(g["plist"]))
+    try:
+        model.set_initial_state_sensitivities(g["sx0"][0])
+        one = _run_kw(model, {k: (v[:1] if k in ("theta", "kfix", "meas", "sigma") else v)
+                              for k, v in g.items()}, meth)
+        np.testing.assert_array_equal(one["sllh"][0], out["sllh"][0])
+    finally:
+        model.set_initial_state_sensitivities([])
+        model.set_parameter_list(range(model.np))

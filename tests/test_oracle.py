@@ -372,3 +372,40 @@ def test_initial_state_override():
     b = O.simulate(net, net.theta_nominal[None], np.zeros((1, 0)), ts, rtol=1e-10, atol=1e-14,
                    x0=net.x0_const[None])
     assert np.array_equal(a["x"], b["x"])
+
+
+def test_initial_state_sensitivities():
+    """ExpData::sx0 (Model::set_initial_state_sensitivities, src/model.cpp:986-1023),
+    given with respect to the scaled parameters: the linear chain's forward
+    sensitivities are expm(A t) sx0 plus the parameters' own contribution
+    (Van Loan, as in test_linear_states_and_fsa_closed_form), and the adjoint
+    gradient with sx0 equals the forward one."""
+    net = networks.linear_chain(3)
+    n = net.nx
+    ts = np.array([0.0, 0.5, 2.0, 6.0])
+    rng = np.random.default_rng(3)
+    sx0 = rng.normal(size=(net.np, n))
+    r = O.simulate(net, net.theta_nominal[None], np.zeros((1, 0)), ts, rtol=1e-12, atol=1e-16,
+                   sensi=1, want_sx=True, sx0=sx0[None])
+    A, k = _linear_A(net, net.theta_nominal)
+    x0 = net.x0_const
+    for it, t in enumerate(ts):
+        for j in range(net.np):
+            dA = np.zeros((n, n))
+            rx = net.reactions[j]
+            (a,), (b,) = rx.reactants, rx.products
+            dA[a, a] -= 1.0
+            dA[b, a] += 1.0
+            big = np.block([[A, dA], [np.zeros((n, n)), A]])
+            dx = (expm(big * t)[:n, n:] @ x0) * k[j] * math.log(10.0) + expm(A * t) @ sx0[j]
+            np.testing.assert_allclose(r["sx"][0, it, j], dx, rtol=1e-7, atol=1e-11)
+    y = O.simulate(net, net.theta_nominal[None], np.zeros((1, 0)), ts, rtol=1e-12, atol=1e-16)["y"][0]
+    meas = y + 0.05
+    sig = np.full_like(meas, 0.1)
+    kw = dict(rtol=1e-12, atol=1e-16, quad_rtol=1e-12, quad_atol=1e-16, sx0=sx0[None])
+    f = O.simulate(net, net.theta_nominal[None], np.zeros((1, 0)), ts, meas[None], sig[None], sensi=1, **kw)
+    a = O.simulate(net, net.theta_nominal[None], np.zeros((1, 0)), ts, meas[None], sig[None], sensi=2, **kw)
+    z = O.simulate(net, net.theta_nominal[None], np.zeros((1, 0)), ts, meas[None], sig[None], sensi=2,
+                   rtol=1e-12, atol=1e-16, quad_rtol=1e-12, quad_atol=1e-16)
+    np.testing.assert_allclose(a["sllh"], f["sllh"], rtol=1e-6, atol=1e-9)
+    assert np.abs(a["sllh"] - z["sllh"]).max() > 1e-3   # sx0 changes the gradient
