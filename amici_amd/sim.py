@@ -493,7 +493,10 @@ def _pscale_list(pscale, n: int) -> list:
 def _as_dev(a, dev, dtype=torch.float64) -> torch.Tensor:
     if isinstance(a, torch.Tensor):
         return a.to(device=dev, dtype=dtype).contiguous()
-    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=dev)
+    a = np.ascontiguousarray(a)
+    if not a.flags.writeable:  # broadcast views: torch wants writable memory
+        a = a.copy()
+    return torch.as_tensor(a, dtype=dtype, device=dev)
 
 
 def _ptr(t):
