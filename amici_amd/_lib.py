@@ -9,7 +9,7 @@ import ctypes
 import math
 from pathlib import Path
 
-AMX_ABI_VERSION = 8
+AMX_ABI_VERSION = 9
 AMX_NSTATS = 18
 AMX_NDIAG = 9
 SENSI_NONE, SENSI_FORWARD, SENSI_ADJOINT = 0, 1, 2
@@ -37,6 +37,7 @@ class SolverOpts(ctypes.Structure):
         ("ss_rtol", ctypes.c_double), ("ss_atol", ctypes.c_double),
         ("max_blocks", ctypes.c_int), ("interp", ctypes.c_int),
         ("stldet", ctypes.c_int), ("failfast", ctypes.c_int),
+        ("ckpt_steps", ctypes.c_int),
     ]
 
 
@@ -137,9 +138,10 @@ def make_opts(rtol=1e-8, atol=1e-16, sensi_meth=SENSI_NONE, maxsteps=10000, rtol
               atol_fsa=math.nan, rtolB=math.nan, atolB=math.nan, quad_rtol=1e-8,
               quad_atol=1e-12, maxstepsB=0, max_traj=0, maxtime=0.0, newton_maxsteps=40,
               ss_rtol=math.nan, ss_atol=math.nan, max_blocks=0,
-              interp=INTERP_POLYNOMIAL, stldet=True, failfast=False) -> SolverOpts:
+              interp=INTERP_POLYNOMIAL, stldet=True, failfast=False,
+              ckpt_steps=0) -> SolverOpts:
     return SolverOpts(rtol, atol, rtol_fsa, atol_fsa, rtolB, atolB, quad_rtol, quad_atol,
                       int(maxsteps), int(maxstepsB), int(sensi_meth), int(max_traj),
                       float(maxtime), int(newton_maxsteps), float(ss_rtol), float(ss_atol),
                       int(max_blocks), int(interp), STLDET_ON if stldet else STLDET_OFF,
-                      1 if failfast else 0)
+                      1 if failfast else 0, int(ckpt_steps))

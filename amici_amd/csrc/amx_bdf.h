@@ -506,6 +506,24 @@ struct SolverParams {
 };
 
 
+// Optional problem properties (false when the problem does not declare them):
+//   P::FORCE_SETUP  the integrator honours force_setup (a fresh Jacobian and
+//                   Newton matrix at the next nonlinear solve)
+//   P::CK_SAVE      after every accepted step the integrator writes its state
+//                   to pb.ck_rec + pb.ck_due * CK_LEN when pb.ck_due >= 0
+template <class P, class = void>
+struct force_setup_of : std::false_type {};
+template <class P>
+struct force_setup_of<P, std::void_t<decltype(P::FORCE_SETUP)>>
+    : std::bool_constant<P::FORCE_SETUP> {};
+template <class P, class = void>
+struct ck_save_of : std::false_type {};
+template <class P>
+struct ck_save_of<P, std::void_t<decltype(P::CK_SAVE)>> : std::bool_constant<P::CK_SAVE> {};
+
+// scalars of an integrator checkpoint after the Nordsieck array (save_state)
+constexpr int CK_SCAL = 72;
+
 // BDF integrator over a Problem P.  P provides:
 //   NX, T, NQ (quadratures, 0 if none), NS (forward sensitivities, 0 if none)
 //   double rhs(double t, double y)                    team-collective, returns f_lane
@@ -522,6 +540,10 @@ struct BDF {
   static constexpr int EPQ = (NQ > 0) ? (NQ + T - 1) / T : 1;
   static constexpr bool QUAD = NQ > 0;
   static constexpr bool SENS = P::NS > 0;
+  static constexpr bool FORCE = force_setup_of<P>::value;
+  static constexpr bool CK_SAVE = ck_save_of<P>::value;
+  // doubles of a checkpoint record: zn[LMAX][NX], then CK_SCAL scalars
+  static constexpr int CK_LEN = LMAX * NX + CK_SCAL;
   using L = BdfLds<NX, NQ>;
 
   P& pb;
@@ -545,6 +567,7 @@ struct BDF {
       tolsf, tstop, dsm_last, dsm_test;
   int q, qprime, L_, qwait, nst, nstlp, nstlj, nscon, convfail;
   bool jcur, tstopset, at_tstop;
+  bool force_setup;  // FORCE problems: fresh Jacobian and Newton matrix at the next solve
   // values returned by advance(): this lane's state and quadratures
   double yout, yQout[EPQ];
   // counters
@@ -558,6 +581,7 @@ struct BDF {
 
   __device__ __forceinline__ BDF(P& p, L& lds, const SolverParams& s) : pb(p), S(lds), sp(s) {
     active = tm.lane < NX;
+    force_setup = false;
     if constexpr (QUAD) nq = p.nq;
     else nq = 0;
     trace = nullptr;
@@ -902,7 +926,9 @@ struct BDF {
     AMX_PF_BEGIN(t0);
     const double dgamma = fabs(gamma / gammap - 1.0);
     const bool jbad = (nst == 0) || (nst >= nstlj + MSBJ) ||
-                      (cf == FAIL_BAD_J && dgamma < DGMAX_JBAD) || (cf == FAIL_OTHER);
+                      (cf == FAIL_BAD_J && dgamma < DGMAX_JBAD) || (cf == FAIL_OTHER) ||
+                      (FORCE && force_setup);
+    if constexpr (FORCE) force_setup = false;
     if (jbad) {
       pb.jac(tn, y);
       jcur = true;
@@ -1003,7 +1029,7 @@ struct BDF {
     convfail = (nflag == FIRST_CALL || nflag == PREV_ERR_FAIL) ? NO_FAILURES : FAIL_OTHER;
     const bool callSetup = (nflag == PREV_CONV_FAIL) || (nflag == PREV_ERR_FAIL) ||
                            (nst == 0) || (nst >= nstlp + MSBP) ||
-                           (fabs(gamrat - 1.0) > DGMAX_LSETUP);
+                           (fabs(gamrat - 1.0) > DGMAX_LSETUP) || (FORCE && force_setup);
     acor = 0.0;
     const int flag = newton(callSetup);
     if (flag != NLS_OK) return flag;
@@ -1580,6 +1606,15 @@ struct BDF {
       const double troundoff = FUZZ_FACTOR * UROUND * (fabs(tn) + fabs(h));
       if (fabs(tn - tstop) <= troundoff) tn = tstop;
       if (!pb.on_step(tn, Z(0), Z(1), hscale, q)) return AMX_TRAJ_FULL;
+      if constexpr (CK_SAVE) {
+        // a checkpoint (CVodeF, cvodea.c:577-597): the state after this step,
+        // and a fresh Newton matrix for the next one
+        if (pb.ck_due >= 0) {
+          save_state(pb.ck_rec + (size_t)pb.ck_due * CK_LEN, pb.jout);
+          pb.ck_due = -1;
+          force_setup = true;
+        }
+      }
       if (fabs(tn - tstop) <= troundoff) {
         if ((tout - tstop) * h >= 0.0 || fabs(tout - tstop) <= troundoff) {
           tstopset = false;
@@ -1627,6 +1662,8 @@ struct BDF {
   // that step, a fresh Jacobian and Newton matrix with nstlp = nstlj = 0);
   // afterwards nst continues from its old value + 1.  A step that already
   // lands on tnext is an error (CV_TSTOP_RETURN is not CV_SUCCESS there).
+  // With checkpoints the step also stops at the start of the current
+  // checkpoint interval, tst >= tnext; landing there is no error.
   __device__ __forceinline__ int reinit_step(double tnext, int mxstep) {
     const int nst_tmp = nst;
     nst = 0;
@@ -1635,6 +1672,80 @@ struct BDF {
     if (at_tstop) return AMX_ERROR;
     nst = nst_tmp + 1;
     return AMX_SUCCESS;
+  }
+  __device__ __forceinline__ int reinit_step(double tnext, double tst, int mxstep) {
+    const int nst_tmp = nst;
+    nst = 0;
+    const int r = advance(tnext, tst, mxstep, true);
+    if (r != AMX_SUCCESS) return r;
+    if (at_tstop && tst == tnext) return AMX_ERROR;
+    nst = nst_tmp + 1;
+    return AMX_SUCCESS;
+  }
+
+  // ---------------- checkpoints (CVAckpntNew / CVAckpntGet, cvodea.c:1876-1990, 2381-2530) ----
+  // The integrator state that the steps after this point depend on: the
+  // Nordsieck array, the step-size history and method coefficients, the
+  // stability-limit data, and every scalar carried from step to step; jout is
+  // the caller's output index.  The Newton matrix is not part of it: the
+  // caller forces a fresh one after the checkpoint, in the original pass and
+  // in a recomputation alike, so both take the same steps.
+  __device__ __forceinline__ void save_state(double* r, int jout) const {
+    static_assert(!QUAD && !SENS, "checkpoints of the state problem only");
+    if (active)
+#pragma unroll
+      for (int j = 0; j < LMAX; ++j) r[j * NX + tm.lane] = S.zn[j][tm.lane];
+    if (tm.lane == 0) {
+      double* c = r + LMAX * NX;
+      int k = 0;
+      for (int j = 0; j <= LMAX; ++j) c[k++] = S.tau[j];
+      for (int j = 0; j < LMAX; ++j) c[k++] = S.l[j];
+      for (int j = 0; j < 6; ++j) c[k++] = S.tq[j];
+      for (int i = 0; i < 5; ++i)
+        for (int m = 0; m < 3; ++m) c[k++] = S.ssdat[i][m];
+      const double v[] = {tn, h, hprime, hscale, eta, etamax, etaq, etaqm1, etaqp1, gamma,
+                          gammap, gamrat, rl1, crate, crateS, delp, acnrm, acnrmS, saved_tq5,
+                          hu, tolsf, tstop, dsm_last, dsm_test,
+                          (double)q, (double)qprime, (double)L_, (double)qwait, (double)nst,
+                          (double)nstlp, (double)nstlj, (double)nscon, (double)convfail,
+                          jcur ? 1.0 : 0.0, tstopset ? 1.0 : 0.0, at_tstop ? 1.0 : 0.0,
+                          (double)jout};
+      static_assert(7 + 6 + 6 + 15 + sizeof(v) / sizeof(double) <= CK_SCAL, "record size");
+      for (unsigned m = 0; m < sizeof(v) / sizeof(double); ++m) c[k++] = v[m];
+    }
+  }
+  // Restores a state written by save_state (counters start at 0); returns jout.
+  __device__ __forceinline__ int load_state(const double* r) {
+    team_sync();
+    if (active)
+#pragma unroll
+      for (int j = 0; j < LMAX; ++j) S.zn[j][tm.lane] = r[j * NX + tm.lane];
+    const double* c = r + LMAX * NX;
+    if (tm.lane == 0) {
+      int k = 0;
+      for (int j = 0; j <= LMAX; ++j) S.tau[j] = c[k++];
+      for (int j = 0; j < LMAX; ++j) S.l[j] = c[k++];
+      for (int j = 0; j < 6; ++j) S.tq[j] = c[k++];
+      for (int i = 0; i < 5; ++i)
+        for (int m = 0; m < 3; ++m) S.ssdat[i][m] = c[k++];
+    }
+    const double* v = c + 7 + 6 + 6 + 15;
+    tn = v[0]; h = v[1]; hprime = v[2]; hscale = v[3]; eta = v[4]; etamax = v[5];
+    etaq = v[6]; etaqm1 = v[7]; etaqp1 = v[8]; gamma = v[9]; gammap = v[10]; gamrat = v[11];
+    rl1 = v[12]; crate = v[13]; crateS = v[14]; delp = v[15]; acnrm = v[16]; acnrmS = v[17];
+    saved_tq5 = v[18]; hu = v[19]; tolsf = v[20]; tstop = v[21]; dsm_last = v[22];
+    dsm_test = v[23];
+    q = (int)v[24]; qprime = (int)v[25]; L_ = (int)v[26]; qwait = (int)v[27];
+    nst = (int)v[28]; nstlp = (int)v[29]; nstlj = (int)v[30]; nscon = (int)v[31];
+    convfail = (int)v[32];
+    jcur = v[33] != 0.0; tstopset = v[34] != 0.0; at_tstop = v[35] != 0.0;
+    const int jout = (int)v[36];
+    team_sync();
+    nfe = nsetups = nje = nni = netf = ncfn = nfQe = nniS = nor = 0;
+    y = Z(0);
+    yout = y;
+    ewt = 0.0; acor = 0.0; ftemp = 0.0;
+    return jout;
   }
 };
 

@@ -42,7 +42,8 @@ struct BatchArgs {
   int* status;             // [nb]
   int* stats;              // [nb][AMX_NSTATS] or null
   int* diag;               // [nb][nt][AMX_NDIAG] per-output-time counters or null
-  // adjoint forward trajectory (Hermite data at every accepted step)
+  // adjoint forward trajectory (Hermite data at every accepted step; with
+  // checkpoints, the steps of one checkpoint interval: max_traj = ck_every + 1)
   double* traj_t;          // [nb][max_traj]
   double* traj_y;          // [nb][max_traj][NX]
   double* traj_f;          // [nb][max_traj][NX] y' (Hermite interpolation) or null
@@ -50,6 +51,11 @@ struct BatchArgs {
   int* traj_n;             // [nb]
   double* dJydx;           // [nb][nt][NX]
   int max_traj;
+  // adjoint checkpoints (ck_every > 0): a record of the integrator state every
+  // ck_every forward steps, ck_pts forward points (steps + 1) at most
+  int ck_every, ck_max, ck_pts;
+  double* ck_rec;          // [nb][ck_max][CK_LEN]
+  double* ck_spill;        // [nb][sizeof(BdfLds<NX, NP>) / 8] backward LDS during recomputation
   double* sens_ws;         // FSA workspace [nb][SENS_SLOTS][NP][NX]
   const double* kfix_pre;  // [nb][NK] pre-equilibration fixed parameters or null
   int newton_maxsteps;     // damped Newton iterations of the pre-equilibration
@@ -250,9 +256,12 @@ struct MAEval {
 
 // ---------------------------------------------------------------------------
 // forward problem, optionally with NS staggered forward sensitivities
-template <class M, int NS_>
+template <class M, int NS_, bool CK = false>
 struct FwdProblem {
   static constexpr int NX = M::NX, T = M::T, NQ = 0, NS = NS_, NP = M::NP;
+  // checkpoints: the integrator writes a record when ck_due >= 0 and takes a
+  // fresh Newton matrix after it
+  static constexpr bool FORCE_SETUP = CK, CK_SAVE = CK;
   static constexpr int JLD = Smem<M>::JLD;
   const Team<T>& tm;
   Smem<M>& s;
@@ -269,6 +278,12 @@ struct FwdProblem {
   double* tf;
   int* tq;
   int ntraj, max_traj;
+  // checkpoints (CK): every ck_K points the record ck_due = point / ck_K is
+  // due, and only the current interval's points are kept (slot point % ck_K);
+  // a recomputation (ck_base >= 0) stores the points ck_base .. ck_base + ck_K
+  // in slots 0 .. ck_K and then stops
+  int ck_K = 0, ck_base = -1, ck_due = -1, jout = 0;
+  double* ck_rec = nullptr;
 
   __device__ __forceinline__ FwdProblem(const Team<T>& t, Smem<M>& sm, double* w)
       : tm(t), s(sm), ev(t, sm), ws(w), tt(nullptr), ty(nullptr), tf(nullptr), tq(nullptr),
@@ -281,16 +296,30 @@ struct FwdProblem {
   // point)
   __device__ __forceinline__ bool on_step(double tn, double z0, double z1, double hs, int qu) {
     if (tt == nullptr) return true;
-    if (ntraj >= max_traj) return false;
+    int slot = ntraj;
+    if constexpr (CK) {
+      if (ck_base >= 0) {
+        slot = ntraj - ck_base;
+      } else {
+        if (ntraj >= max_traj) return false;
+        slot = ntraj % ck_K;
+        if (ntraj > 0 && slot == 0) ck_due = ntraj / ck_K;  // CVodeF, cvodea.c:577-597
+      }
+    } else {
+      if (ntraj >= max_traj) return false;
+    }
     if (tm.lane == 0) {
-      tt[ntraj] = tn;
-      if (tq) tq[ntraj] = qu;
+      tt[slot] = tn;
+      if (tq) tq[slot] = qu;
     }
     if (tm.lane < NX) {
-      ty[(size_t)ntraj * NX + tm.lane] = z0;
-      if (tf) tf[(size_t)ntraj * NX + tm.lane] = z1 / hs;
+      ty[(size_t)slot * NX + tm.lane] = z0;
+      if (tf) tf[(size_t)slot * NX + tm.lane] = z1 / hs;
     }
     ntraj++;
+    if constexpr (CK) {
+      if (ck_base >= 0 && ntraj - ck_base > ck_K) return false;  // interval complete
+    }
     return true;
   }
 
@@ -685,9 +714,12 @@ struct TrajInterp<M, false> {
 
 // adjoint problem: xB' = -J(x(t))^T xB, qB' = -(df/dp)^T xB, x(t) from the
 // stored forward steps (TrajInterp)
-template <class M, bool POLY>
+template <class M, bool POLY, bool CK = false>
 struct BwdProblem {
   static constexpr int NX = M::NX, T = M::T, NQ = M::NP, NS = 0, NP = M::NP;
+  // checkpoints: a fresh Newton matrix after every recomputation, which
+  // reuses this problem's Jacobian storage
+  static constexpr bool FORCE_SETUP = CK;
   static constexpr int JLD = Smem<M>::JLD;
   static constexpr int EPQ = (NQ + T - 1) / T;
   const Team<T>& tm;
@@ -986,12 +1018,15 @@ __device__ __forceinline__ int steady_state(const BatchArgs& a, const Team<M::T>
 
 // ---------------------------------------------------------------------------
 // forward simulation of one instance.  MODE 0: states, 1: FSA, 2: adjoint
-// forward pass (stores the trajectory and dJy/dx at every output).
+// forward pass (stores the trajectory and dJy/dx at every output), 3: adjoint
+// forward pass with checkpoints (a record of the integrator state every
+// ck_every steps, the last interval's points).
 template <class M, int MODE>
 __device__ __forceinline__ void simulate_forward(int inst, const BatchArgs& a, const Team<M::T>& tm,
                                                  Smem<M>& s, BdfLds<M::NX, 0>& lds) {
   constexpr int NX = M::NX, NP = M::NP, NK = M::NK, NY = M::NY, T = M::T;
   constexpr int NS = (MODE == 1) ? NP : 0;
+  constexpr bool ADJ = MODE >= 2, CKM = MODE == 3;
   const size_t I = (size_t)inst;
   for (int r = tm.lane; r < M::NR; r += T)
     s.pk[r] = unscale_p(a.theta[I * NP + M::KI[r]], a.scale(M::KI[r]));
@@ -1025,30 +1060,36 @@ __device__ __forceinline__ void simulate_forward(int inst, const BatchArgs& a, c
       const int kx = M::X0K[tm.lane];
       x0 = (kx >= 0) ? a.kfix[I * NK + kx] : xp;
     }
-    if constexpr (MODE == 2) {
+    if constexpr (ADJ) {
       if (tm.lane < NX) a.xss[I * NX + tm.lane] = xp;
     }
   }
   double* ws = (MODE == 1) ? a.sens_ws + I * (size_t)SENS_SLOTS * NP * NX : nullptr;
-  using PB = FwdProblem<M, NS>;
+  using PB = FwdProblem<M, NS, CKM>;
   PB pb0(tm, s, ws);
   pb0.ns = (MODE == 1) ? a.npl(NP) : 0;
   pb0.plist = a.plist;
   const int npl = a.npl(NP);
   pb0.ssp_rtol = a.sp.rtolS;
   pb0.ssp_atol = a.sp.atolS;
-  if constexpr (MODE == 2) {
+  if constexpr (ADJ) {
     pb0.tt = a.traj_t + I * (size_t)a.max_traj;
     pb0.ty = a.traj_y + I * (size_t)a.max_traj * NX;
     pb0.tf = a.traj_f ? a.traj_f + I * (size_t)a.max_traj * NX : nullptr;
     pb0.tq = a.traj_q ? a.traj_q + I * (size_t)a.max_traj : nullptr;
     pb0.max_traj = a.max_traj;
   }
+  if constexpr (CKM) {
+    pb0.max_traj = a.ck_pts;
+    pb0.ck_K = a.ck_every;
+    pb0.ck_rec = a.ck_rec + I * (size_t)a.ck_max * BDF<PB>::CK_LEN;
+  }
   pb0.mat = mat;
   PB& pb = pb0;
   BDF<PB> bdf(pb, lds, a.sp);
   if (a.maxtime_ticks) bdf.sp.deadline = wall_clock64() + a.maxtime_ticks;
   bdf.init(0.0, x0, nullptr);
+  if constexpr (CKM) bdf.save_state(pb.ck_rec, 0);  // checkpoint 0: the initial state
   if (a.trace && inst == a.trace_inst) {
     bdf.trace = a.trace;
     bdf.trace_cap = a.trace_cap / 2;  // second half: the backward problem's records
@@ -1097,6 +1138,7 @@ __device__ __forceinline__ void simulate_forward(int inst, const BatchArgs& a, c
         status = AMX_ERROR;
         break;
       }
+      if constexpr (CKM) pb.jout = it;
       status = bdf.advance(tout, tout, bdf.sp.maxsteps - bdf.nst);
       if (status != AMX_SUCCESS) break;
     }
@@ -1119,7 +1161,7 @@ __device__ __forceinline__ void simulate_forward(int inst, const BatchArgs& a, c
                                       a.y ? a.y + mo : nullptr);
     llh -= ob.nllh;
     const double dJdx = ob.dJdx;
-    if constexpr (MODE == 2) {
+    if constexpr (ADJ) {
       if (tm.lane < NX) a.dJydx[(I * a.nt + it) * NX + tm.lane] = dJdx;
     }
     if constexpr (MODE == 1) {
@@ -1150,7 +1192,7 @@ __device__ __forceinline__ void simulate_forward(int inst, const BatchArgs& a, c
     a.llh[inst] = llh;
     a.status[inst] = status;
     if (a.stats) store_stats(a.stats + I * AMX_NSTATS, bdf, 0);
-    if constexpr (MODE == 2) a.traj_n[inst] = pb.ntraj;
+    if constexpr (ADJ) a.traj_n[inst] = pb.ntraj;
   }
   if constexpr (MODE == 1) {
     // sllh and sx: chain rule of each parameter's scale on the non-zero
@@ -1171,14 +1213,66 @@ __device__ __forceinline__ void simulate_forward(int inst, const BatchArgs& a, c
   }
 }
 
+// Recomputation of checkpoint interval c of one instance (CVAdataStore,
+// cvodea.c:2328-2375): the forward problem restarts from record c and stores
+// the points c * K .. (c + 1) * K in the trajectory buffer.  Record 0 is the
+// initial state, so interval 0 repeats the start of the forward pass; a later
+// record restarts inside the forward pass's call towards output jout with a
+// fresh Newton matrix, as the forward pass did after writing it, so the
+// recomputed steps are the forward pass's.  The problem's LDS block is the
+// backward problem's, which the caller saved.
+template <class M>
+__device__ __attribute__((noinline)) int recompute_interval(const BatchArgs& a, const Team<M::T>& tm,
+                                                           Smem<M>& s, BdfLds<M::NX, 0>& lds,
+                                                           int inst, int c,
+                                                           unsigned long long deadline) {
+  constexpr int NX = M::NX;
+  using PR = FwdProblem<M, 0, true>;
+  const size_t I = (size_t)inst;
+  PR pr(tm, s, nullptr);
+  pr.ns = 0;
+  pr.tt = a.traj_t + I * (size_t)a.max_traj;
+  pr.ty = a.traj_y + I * (size_t)a.max_traj * NX;
+  pr.tf = a.traj_f ? a.traj_f + I * (size_t)a.max_traj * NX : nullptr;
+  pr.tq = a.traj_q ? a.traj_q + I * (size_t)a.max_traj : nullptr;
+  pr.max_traj = a.max_traj;
+  pr.ck_K = a.ck_every;
+  pr.ck_base = c * a.ck_every;
+  pr.ntraj = pr.ck_base;
+  pr.mat = big_model<M>() ? a.lu_ws + I * (size_t)NX * NX : nullptr;
+  BDF<PR> bdf(pr, lds, a.sp);
+  bdf.sp.deadline = deadline;
+  const int jout = bdf.load_state(a.ck_rec + (I * (size_t)a.ck_max + c) * BDF<PR>::CK_LEN);
+  if (c > 0) {
+    bdf.force_setup = true;
+    pr.on_step(bdf.tn, bdf.Z(0), bdf.Z(1), bdf.hscale, bdf.q);
+  }
+  const double* ts = a.ts + (size_t)a.ts_stride * I;
+  for (int j = jout; j < a.nt; ++j) {
+    const double tout = ts[j];
+    if (tout > 0.0 && bdf.tn < tout && !next_t_too_close(bdf.tn, tout)) {
+      const int r = bdf.advance(tout, tout, 0x3fffffff);
+      if (r == AMX_TRAJ_FULL) return AMX_SUCCESS;  // point (c + 1) * K stored
+      if (r != AMX_SUCCESS) return r;
+    }
+  }
+  return AMX_ERR_FAILURE;  // the forward pass went on past this interval
+}
+
 // backward (adjoint) sweep of one instance; EventHandlingBwdSimulator::run
 // (src/backwardproblem.cpp:267-332) with a fresh order-1 restart after every
 // data-point jump (CVodeSolver::reinit_b, src/solver_cvodes.cpp:594-649).
-template <class M, bool POLY>
+// With checkpoints (CK) the sweep runs interval by interval as CVodeB does
+// (cvodea.c:1490-1600): each call integrates with the stop time at the
+// current interval's start, and the interval before it is recomputed when the
+// sweep reaches that start.  lds_f is the same LDS block seen as the forward
+// problem's; the backward problem's contents are kept in ck_spill meanwhile.
+template <class M, bool POLY, bool CK = false>
 __device__ __forceinline__ void simulate_backward(int inst, const BatchArgs& a, const Team<M::T>& tm,
-                                                  Smem<M>& s, BdfLds<M::NX, M::NP>& lds) {
+                                                  Smem<M>& s, BdfLds<M::NX, M::NP>& lds,
+                                                  BdfLds<M::NX, 0>* lds_f = nullptr) {
   constexpr int NX = M::NX, NP = M::NP, T = M::T;
-  using PB = BwdProblem<M, POLY>;
+  using PB = BwdProblem<M, POLY, CK>;
   constexpr int EPQ = PB::EPQ;
   const size_t I = (size_t)inst;
   int* const diag = a.diag ? a.diag + I * a.nt * AMX_NDIAG : nullptr;
@@ -1197,9 +1291,18 @@ __device__ __forceinline__ void simulate_backward(int inst, const BatchArgs& a, 
   team_sync();
   const int n = a.traj_n[inst];
   const double* tt = a.traj_t + I * (size_t)a.max_traj;
-  PB pb0(tm, s, tt, a.traj_y + I * (size_t)a.max_traj * NX,
-         POLY ? a.traj_q + I * (size_t)a.max_traj : nullptr,
-         POLY ? nullptr : a.traj_f + I * (size_t)a.max_traj * NX, n);
+  const double* ty = a.traj_y + I * (size_t)a.max_traj * NX;
+  const int* tqo = POLY ? a.traj_q + I * (size_t)a.max_traj : nullptr;
+  const double* tfd = POLY ? nullptr : a.traj_f + I * (size_t)a.max_traj * NX;
+  // checkpoints: interval c spans the points c * K .. (c + 1) * K; the
+  // forward pass left the last one, of n_last points, in the buffer
+  const int K = CK ? a.ck_every : 0;
+  int c = 0, n_last = n;
+  if constexpr (CK) {
+    c = (n > 0) ? (n - 1) / K : 0;
+    n_last = n - c * K;
+  }
+  PB pb0(tm, s, tt, ty, tqo, tfd, n_last);
   pb0.nq = npl;
   pb0.plist = a.plist;
   constexpr bool BIG = big_model<M>();
@@ -1214,7 +1317,7 @@ __device__ __forceinline__ void simulate_backward(int inst, const BatchArgs& a, 
   const double* dJ = a.dJydx + I * a.nt * NX;
   const double t0 = 0.0;
   int it = a.nt - 1;
-  double tcur = (n > 0) ? tt[n - 1] : t0;
+  double tcur = (n > 0) ? tt[n_last - 1] : t0;
   double xB = 0.0;
   double qB[EPQ];
 #pragma unroll
@@ -1238,17 +1341,66 @@ __device__ __forceinline__ void simulate_backward(int inst, const BatchArgs& a, 
   // integrated before, takes its first step through reInit_post_process with
   // the stop time at the next data point.
   const int mxB = a.spB.maxsteps;
+  // start time of checkpoint interval k (record k's time; t0 for k = 0)
+  const double* ckr = CK ? a.ck_rec + I * (size_t)a.ck_max * BDF<PB>::CK_LEN : nullptr;
+  auto ck_t = [&](int k) -> double {
+    return k > 0 ? ckr[(size_t)k * BDF<PB>::CK_LEN + LMAX * NX + 7 + 6 + 6 + 15] : t0;
+  };
+  // moves to the interval before the current one while the backward problem
+  // is at or before the current one's start, recomputing it
+  auto enter = [&]() -> int {
+    if constexpr (CK) {
+      for (int g = 0; g < n && c > 0 && bdf.tn <= ck_t(c); ++g) {
+        --c;
+        double* spill = a.ck_spill + I * (size_t)(sizeof(BdfLds<NX, NP>) / sizeof(double));
+        double* l64 = reinterpret_cast<double*>(&lds);
+        constexpr int NW = (int)(sizeof(BdfLds<NX, NP>) / sizeof(double));
+        team_sync();
+        for (int k = tm.lane; k < NW; k += T) spill[k] = l64[k];
+        team_sync();
+        const int r = recompute_interval<M>(a, tm, s, *lds_f, inst, c, bdf.sp.deadline);
+        team_sync();
+        for (int k = tm.lane; k < NW; k += T) l64[k] = spill[k];
+        team_sync();
+        if (r != AMX_SUCCESS) return r;
+        pb0.ip.init(tt, ty, tqo, tfd, K + 1);
+        pb0.have_cached = false;
+        bdf.force_setup = true;
+      }
+    }
+    return AMX_SUCCESS;
+  };
+  // CVode towards tnext, across checkpoint intervals (CK)
+  auto advance_to = [&](double tnext) -> int {
+    for (int g = 0; g <= n; ++g) {
+      int r = enter();
+      if (r != AMX_SUCCESS) return r;
+      const double tst = ck_t(c);
+      r = bdf.advance(tnext, tst, mxB);
+      if (r != AMX_SUCCESS) return r;
+      if (!bdf.at_tstop || tst <= tnext) return AMX_SUCCESS;
+    }
+    return AMX_ERR_FAILURE;
+  };
+  auto reinit_to = [&](double tnext) -> int {
+    const int r = enter();
+    if (r != AMX_SUCCESS) return r;
+    const double tst = ck_t(c) > tnext ? ck_t(c) : tnext;
+    return bdf.reinit_step(tnext, tst, mxB);
+  };
   bool called = false, force_pp = false;
   bdf.init(tcur, xB, qB);
   while (it >= 0 && status == AMX_SUCCESS) {
     const double tnext = ts[it];
     if (tnext < tcur) {
       if (force_pp) {
-        status = bdf.reinit_step(tnext, mxB);
+        if constexpr (CK) status = reinit_to(tnext);
+        else status = bdf.reinit_step(tnext, mxB);
         force_pp = false;
         if (status != AMX_SUCCESS) break;
       }
-      status = bdf.advance(tnext, t0, mxB);
+      if constexpr (CK) status = advance_to(tnext);
+      else status = bdf.advance(tnext, t0, mxB);
       if (status != AMX_SUCCESS) break;
       called = true;
       tcur = tnext;
@@ -1274,8 +1426,14 @@ __device__ __forceinline__ void simulate_backward(int inst, const BatchArgs& a, 
     }
   }
   if (status == AMX_SUCCESS && tcur > t0) {
-    if (force_pp) status = bdf.reinit_step(t0, mxB);
-    if (status == AMX_SUCCESS) status = bdf.advance(t0, t0, mxB);
+    if (force_pp) {
+      if constexpr (CK) status = reinit_to(t0);
+      else status = bdf.reinit_step(t0, mxB);
+    }
+    if (status == AMX_SUCCESS) {
+      if constexpr (CK) status = advance_to(t0);
+      else status = bdf.advance(t0, t0, mxB);
+    }
     if (status == AMX_SUCCESS) {
       xB = bdf.yout;
 #pragma unroll
@@ -1437,6 +1595,30 @@ __device__ __forceinline__ void backward_kernel_body(BatchArgs a) {
     if (inst >= a.nb) break;
     if (skip_failfast<M>(a, tm, inst)) continue;
     simulate_backward<M, POLY>(inst, a, tm, s, lds);
+    note_failure(a, tm, inst);
+  }
+}
+
+// backward sweep with checkpoints: the LDS block of a team serves the
+// backward problem and, during a recomputation, the forward problem
+template <class M, bool POLY>
+__device__ __forceinline__ void backward_ck_kernel_body(BatchArgs a) {
+  constexpr int T = M::T;
+  union Blk {
+    BdfLds<M::NX, M::NP> b;
+    BdfLds<M::NX, 0> f;
+  };
+  static_assert(sizeof(BdfLds<M::NX, 0>) <= sizeof(BdfLds<M::NX, M::NP>), "LDS block");
+  __shared__ Smem<M> smem[Launch<M>::TEAMS];
+  __shared__ Blk blds[Launch<M>::TEAMS];
+  Team<T> tm;
+  Smem<M>& s = smem[threadIdx.x / T];
+  Blk& blk = blds[threadIdx.x / T];
+  for (int guard = 0; guard <= a.nb; ++guard) {
+    const int inst = next_instance<M>(tm, a.counter);
+    if (inst >= a.nb) break;
+    if (skip_failfast<M>(a, tm, inst)) continue;
+    simulate_backward<M, POLY, true>(inst, a, tm, s, blk.b, &blk.f);
     note_failure(a, tm, inst);
   }
 }

@@ -271,6 +271,7 @@ class Solver:
     max_blocks: int = 0  # launch shape: 0 = as many blocks as fit the device at once
     interp: int = 2      # InterpolationType: 1 hermite, 2 polynomial (AMICI's default)
     stldet: bool = True  # BDF stability-limit detection (AMICI's default: on)
+    ckpt_steps: int = 0  # adjoint: checkpoint every K forward steps (0: store every step)
 
     def set_relative_tolerance(self, v): self.rtol = float(v)
     def get_relative_tolerance(self): return self.rtol
@@ -322,6 +323,22 @@ class Solver:
     def get_stability_limit_flag(self):
         return self.stldet
 
+    def set_checkpoint_steps(self, v):
+        """Adjoint checkpoints every ``v`` forward steps (CVodeAdjInit(Nd),
+        cvodea.c:99): the forward pass keeps a checkpoint of the integrator
+        state every ``v`` steps instead of every step, and the backward sweep
+        recomputes each interval from its checkpoint.  0 (the default) stores
+        every step, which is what AMICI does (CVodeAdjInit(maxsteps + 1),
+        src/solver_cvodes.cpp:726-739).  The integrator takes a fresh Newton
+        matrix at each checkpoint, so results differ from the default within
+        the integration tolerances."""
+        if int(v) < 0:
+            raise ValueError("checkpoint steps must be >= 0")
+        self.ckpt_steps = int(v)
+
+    def get_checkpoint_steps(self):
+        return self.ckpt_steps
+
     def set_max_blocks(self, v):
         """Caps the blocks of the persistent kernels (no counterpart in the
         reference): with fewer blocks than instances each block takes further
@@ -348,7 +365,7 @@ class Solver:
                               maxtime=self.maxtime, newton_maxsteps=self.newton_maxsteps,
                               ss_rtol=self.ss_rtol, ss_atol=self.ss_atol,
                               max_blocks=self.max_blocks, interp=self.interp,
-                              stldet=self.stldet)
+                              stldet=self.stldet, ckpt_steps=self.ckpt_steps)
 
 
 class ExpData:

@@ -80,9 +80,10 @@ CONFIGS = {
                 workload="N40 adjoint NLL+gradient: 40 species, 120 reactions, 1024 input "
                          "conditions, 25 times, 30 log-normal observables"),
     "N200": dict(counters=("n200_conditions4.npz", "counters_bench"), cpu_sample=16,
-                 plain_sample=0, traffic="n200_adjoint_hbm_traffic.json",
+                 plain_sample=0, traffic="n200_adjoint_ck32_hbm_traffic.json", ckpt=32,
                  workload="N200 adjoint NLL+gradient: 200 species, 600 reactions, 256 input "
-                          "conditions, 40 times, 60 log-normal observables"),
+                          "conditions, 40 times, 60 log-normal observables, forward "
+                          "checkpoints every 32 steps with recomputation"),
 }
 
 
@@ -192,6 +193,7 @@ def main() -> int:
     adj = sim.Solver(**TOL)
     adj.set_sensitivity_order(sim.SensitivityOrder.first)
     adj.set_sensitivity_method(sim.SensitivityMethod.adjoint)
+    adj.set_checkpoint_steps(cfg.get("ckpt", 0))
 
     # the timed loop replays the batch captured once as a HIP graph
     # (counter reset, forward kernel, backward kernel)
@@ -255,7 +257,8 @@ def main() -> int:
                 "issued": "FP64 VALU (this kernel issues no MFMA instruction)",
                 "limiter": "per-instance dependent-instruction latency (one wavefront per SIMD)",
                 "traffic": (tr or {}).get("bytes_per_launch") if tr else None,
-                "kernel": "amx_bwd_adj" if dom == "bwd" else "amx_fwd_adj",
+                "kernel": ("amx_bwd_adj" if dom == "bwd" else "amx_fwd_adj")
+                          + ("_ck" if cfg.get("ckpt") else ""),
                 "kernel_ms": dom_ms, "flops_per_launch": dom_flops,
                 "fwd_kernel_ms": fwd_ms, "bwd_kernel_ms": bwd_ms,
                 "fwd_flops_per_launch": float(f_fwd.sum()),
@@ -277,7 +280,7 @@ def main() -> int:
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": cfg["workload"],
                    "instances": nb, "np": net.np, "rtol": TOL["rtol"], "atol": TOL["atol"],
-                   "sensitivity": "adjoint"},
+                   "sensitivity": "adjoint", "checkpoint_steps": cfg.get("ckpt", 0)},
         "instances_ok": n_ok,
         "plain_gpu_bdf": plain,
         "speedup_vs_plain_gpu_bdf": (value / plain["value"]) if plain else None,

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define AMX_ABI_VERSION 8
+#define AMX_ABI_VERSION 9
 #define AMX_NSTATS 18
 #define AMX_NDIAG 9
 /* bytes of the static workspace of amx_run_batch_static, per GPU */
@@ -76,6 +76,18 @@ typedef struct {
                                   (src/amici.cpp:232-260); which later instances had already
                                   started depends on timing, so a caller reports every
                                   instance after the first failed one as not run */
+    int ckpt_steps;            /* adjoint: 0 (AMICI's setting, CVodeAdjInit(maxsteps + 1),
+                                  src/solver_cvodes.cpp:726-739): every forward step is
+                                  stored; K > 0: the forward pass writes a checkpoint of
+                                  the integrator state every K steps and keeps only the
+                                  last interval's steps, and the backward sweep
+                                  recomputes each earlier interval from its checkpoint
+                                  (CVodeAdjInit(K), CVodeF / CVAdataStore / CVodeB,
+                                  cvodea.c:395-660, 1365-1630, 2328-2375).  The integrator
+                                  takes a fresh Jacobian at every checkpoint (forward)
+                                  and after every recomputation (backward), so the
+                                  recomputed steps equal the forward pass's; max_traj
+                                  then bounds the forward steps + 1 covered */
 } amx_solver_opts;
 
 typedef struct {

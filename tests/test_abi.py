@@ -163,24 +163,61 @@ def test_teams_narrower_than_a_wavefront_are_refused():
         codegen.emit_model_traits(networks.network_n40(), 32)
 
 
-def test_abi_v8_struct_layout():
-    """amx_solver_opts ends with max_blocks, interp, stldet and failfast,
-    amx_batch_in with the parameter list, scales, initial states and their
-    sensitivities, amx_batch_out with diag; 18 counters per instance
-    (include/amx.h, ABI version 8)."""
-    assert _lib.AMX_ABI_VERSION == 8 and _lib.AMX_NSTATS == 18
+def test_abi_v9_struct_layout():
+    """amx_solver_opts ends with max_blocks, interp, stldet, failfast and
+    ckpt_steps, amx_batch_in with the parameter list, scales, initial states
+    and their sensitivities, amx_batch_out with diag; 18 counters per instance
+    (include/amx.h, ABI version 9)."""
+    assert _lib.AMX_ABI_VERSION == 9 and _lib.AMX_NSTATS == 18
     assert [f[0] for f in _lib.BatchIn._fields_[-5:]] == ["nplist", "plist", "pscale", "x0", "sx0"]
-    assert [f[0] for f in _lib.SolverOpts._fields_[-4:]] == ["max_blocks", "interp", "stldet",
-                                                             "failfast"]
+    assert [f[0] for f in _lib.SolverOpts._fields_[-5:]] == ["max_blocks", "interp", "stldet",
+                                                             "failfast", "ckpt_steps"]
     assert _lib.make_opts().failfast == 0 and _lib.make_opts(failfast=True).failfast == 1
+    assert _lib.make_opts().ckpt_steps == 0 and _lib.make_opts(ckpt_steps=32).ckpt_steps == 32
     assert _lib.BatchOut._fields_[-1][0] == "diag"
     text = (ROOT / "include" / "amx.h").read_text()
-    assert "#define AMX_ABI_VERSION 8" in text and "#define AMX_NDIAG 9" in text
+    assert "#define AMX_ABI_VERSION 9" in text and "#define AMX_NDIAG 9" in text
     assert "#define AMX_NSTATS 18" in text
     assert "#define AMX_INTERP_HERMITE 1" in text and "#define AMX_INTERP_POLYNOMIAL 2" in text
     assert "#define AMX_STLDET_ON 1" in text and "#define AMX_STLDET_OFF 2" in text
     assert _lib.make_opts().stldet == _lib.STLDET_ON
     assert _lib.make_opts(stldet=False).stldet == _lib.STLDET_OFF
+
+
+def test_checkpoint_workspace(lib):
+    """With a checkpoint every K forward steps the workspace holds one
+    interval of K + 1 points, a record of 6 nx + 72 doubles per checkpoint and
+    the backward problem's LDS block per instance, instead of every step."""
+    net, L = lib
+    nb, K = 8, 32
+    full = _lib.make_opts(sensi_meth=_lib.SENSI_ADJOINT, maxsteps=1000)
+    ck = _lib.make_opts(sensi_meth=_lib.SENSI_ADJOINT, maxsteps=1000, ckpt_steps=K)
+    per_pt = (1 + net.nx) * 8 + 4
+    rec = nb * (1000 // K + 1) * (6 * net.nx + 72) * 8
+    d = L.workspace_bytes(full, nb, 10) - L.workspace_bytes(ck, nb, 10)
+    spill = nb * (1001 - (K + 1)) * per_pt - rec - d
+    assert 0 < spill <= nb * 65536 + 4096, spill
+    # states and forward sensitivities ignore the setting
+    for m in (_lib.SENSI_NONE, _lib.SENSI_FORWARD):
+        assert (L.workspace_bytes(_lib.make_opts(sensi_meth=m, ckpt_steps=K), nb, 10)
+                == L.workspace_bytes(_lib.make_opts(sensi_meth=m), nb, 10))
+    bad = _lib.make_opts(sensi_meth=_lib.SENSI_ADJOINT, ckpt_steps=-1)
+    buf = (ctypes.c_double * 4)()
+    p = ctypes.cast(buf, ctypes.c_void_p).value
+    ins = _lib.BatchIn(1, 3, 0, p, p if net.nk else None, p, None, p, None)
+    outs = _lib.BatchOut(None, None, None, p, p, p, None, None)
+    assert L.lib.amx_run_batch(ctypes.byref(bad), ctypes.byref(ins), ctypes.byref(outs), p, 8,
+                               None) == -22
+
+
+def test_solver_checkpoint_setting():
+    from amici_amd import sim
+    s = sim.Solver()
+    assert s.get_checkpoint_steps() == 0 and s.c_opts().ckpt_steps == 0
+    s.set_checkpoint_steps(32)
+    assert s.c_opts().ckpt_steps == 32
+    with pytest.raises(ValueError):
+        s.set_checkpoint_steps(-1)
 
 
 def test_invalid_stldet_rejected(lib):

@@ -393,3 +393,57 @@ def test_initial_state_sensitivities(models, meth):
     finally:
         model.set_initial_state_sensitivities([])
         model.set_parameter_list(range(model.np))
+
+
+@pytest.mark.parametrize("model,fixture,k,interp", [("robertson", "robertson_batch64", 32, 2),
+                                                    ("robertson", "robertson_batch64", 5, 1),
+                                                    ("n40", "n40_conditions16", 32, 2),
+                                                    ("n40", "n40_conditions16", 7, 1)])
+def test_adjoint_checkpoints(models, model, fixture, k, interp):
+    """Adjoint with a checkpoint every k forward steps and each interval
+    recomputed in the backward sweep (CVodeAdjInit(k), cvodea.c:99, with
+    CVodeB's interval-wise sweep): llh and gradient against the oracle, which
+    stores every step as AMICI does, at the parity bar; polynomial (2) and
+    Hermite (1) interpolation."""
+    from amici_amd import sim
+    g = _load(fixture)
+    out = _run(models, model, g, sim.SensitivityMethod.adjoint, ckpt_steps=k, interp=interp)
+    _dump(f"asa_ck{k}_{fixture}", out)
+    assert (out["status"] == 0).all(), out["status"]
+    # every instance spans several checkpoint intervals
+    assert (out["stats"][:, 0] > 3 * k).all(), out["stats"][:, 0]
+    _check_llh(models[model].network, out, g)
+    _close(out["sllh"], g["sllh_asa"], 1e-5, 1e-7, f"sllh (adjoint, checkpoints every {k})")
+
+
+@pytest.mark.parametrize("interp", [2, 1])
+def test_adjoint_checkpoint_spacing_beyond_steps(models, interp):
+    """A checkpoint spacing of max_traj (no checkpoint after the initial one,
+    so nothing is recomputed) reproduces the run that stores every step: the
+    forward pass bit for bit (llh, forward counters), the gradient to 1e-7
+    relative (the backward kernels are compiled separately, and the compiler's
+    contraction of multiply-adds may differ between them)."""
+    from amici_amd import sim
+    g = _load("n40_conditions16")
+    base = _run(models, "n40", g, sim.SensitivityMethod.adjoint, interp=interp)
+    ck = _run(models, "n40", g, sim.SensitivityMethod.adjoint, interp=interp,
+              ckpt_steps=TIGHT["maxsteps"] + 1)
+    assert (base["status"] == 0).all() and (ck["status"] == 0).all()
+    assert np.array_equal(base["llh"], ck["llh"])
+    assert np.array_equal(base["stats"][:, :8], ck["stats"][:, :8])
+    _close(ck["sllh"], base["sllh"], 1e-7, 1e-9, "sllh (checkpoint kernels vs default)")
+
+
+def test_n200_adjoint_checkpoints(n200_model):
+    """The 200-species network with a checkpoint every 32 forward steps and
+    recomputation (one instance per workgroup, the Newton matrix of the
+    recomputation in the backward problem's global-memory slot) against the
+    oracle's gradient."""
+    from amici_amd import sim
+    g = _load("n200_conditions4")
+    out = _run({"m": n200_model}, "m", g, sim.SensitivityMethod.adjoint, ckpt_steps=32)
+    _dump("asa_ck32_n200", out)
+    assert (out["status"] == 0).all(), out["status"]
+    assert (out["stats"][:, 0] > 3 * 32).all()
+    _check_llh(n200_model.network, out, g)
+    _close(out["sllh"], g["sllh_asa"], 1e-5, 1e-7, "sllh (adjoint, checkpoints every 32)")
