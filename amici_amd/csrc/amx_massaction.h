@@ -1221,11 +1221,15 @@ __device__ __forceinline__ void simulate_forward(int inst, const BatchArgs& a, c
 // fresh Newton matrix, as the forward pass did after writing it, so the
 // recomputed steps are the forward pass's.  The problem's LDS block is the
 // backward problem's, which the caller saved.
+// With a step trace (trace != nullptr) the recomputed steps are appended to
+// it from record trace_n on.
 template <class M>
 __device__ __attribute__((noinline)) int recompute_interval(const BatchArgs& a, const Team<M::T>& tm,
                                                            Smem<M>& s, BdfLds<M::NX, 0>& lds,
                                                            int inst, int c,
-                                                           unsigned long long deadline) {
+                                                           unsigned long long deadline,
+                                                           double* trace, int trace_cap,
+                                                           int& trace_n) {
   constexpr int NX = M::NX;
   using PR = FwdProblem<M, 0, true>;
   const size_t I = (size_t)inst;
@@ -1242,21 +1246,32 @@ __device__ __attribute__((noinline)) int recompute_interval(const BatchArgs& a, 
   pr.mat = big_model<M>() ? a.lu_ws + I * (size_t)NX * NX : nullptr;
   BDF<PR> bdf(pr, lds, a.sp);
   bdf.sp.deadline = deadline;
+  bdf.trace = trace;
+  bdf.trace_cap = trace_cap;
+  bdf.trace_n = trace_n;
   const int jout = bdf.load_state(a.ck_rec + (I * (size_t)a.ck_max + c) * BDF<PR>::CK_LEN);
   if (c > 0) {
     bdf.force_setup = true;
     pr.on_step(bdf.tn, bdf.Z(0), bdf.Z(1), bdf.hscale, bdf.q);
   }
   const double* ts = a.ts + (size_t)a.ts_stride * I;
+  int r = AMX_ERR_FAILURE;  // the forward pass went on past this interval
   for (int j = jout; j < a.nt; ++j) {
     const double tout = ts[j];
     if (tout > 0.0 && bdf.tn < tout && !next_t_too_close(bdf.tn, tout)) {
-      const int r = bdf.advance(tout, tout, 0x3fffffff);
-      if (r == AMX_TRAJ_FULL) return AMX_SUCCESS;  // point (c + 1) * K stored
-      if (r != AMX_SUCCESS) return r;
+      const int rr = bdf.advance(tout, tout, 0x3fffffff);
+      if (rr == AMX_TRAJ_FULL) {  // point (c + 1) * K stored
+        r = AMX_SUCCESS;
+        break;
+      }
+      if (rr != AMX_SUCCESS) {
+        r = rr;
+        break;
+      }
     }
   }
-  return AMX_ERR_FAILURE;  // the forward pass went on past this interval
+  trace_n = bdf.trace_n;
+  return r;
 }
 
 // backward (adjoint) sweep of one instance; EventHandlingBwdSimulator::run
@@ -1309,9 +1324,19 @@ __device__ __forceinline__ void simulate_backward(int inst, const BatchArgs& a, 
   if constexpr (BIG) pb0.mat = a.lu_ws + I * (size_t)NX * NX;
   BDF<PB> bdf(pb0, lds, a.spB);  // holds pb0 and the LDS block by reference
   if (a.maxtime_ticks) bdf.sp.deadline = wall_clock64() + a.maxtime_ticks;
+  // step trace: the backward steps in the second half of the buffer; with
+  // checkpoints the second half's first half, and the recomputed forward
+  // steps in its second half
+  double* rtrace = nullptr;
+  int rtrace_cap = 0, rtrace_n = 0;
   if (a.trace && inst == a.trace_inst) {
     bdf.trace = a.trace + (size_t)(a.trace_cap / 2) * TRACE_REC;
     bdf.trace_cap = a.trace_cap - a.trace_cap / 2;
+    if constexpr (CK) {
+      rtrace_cap = bdf.trace_cap - bdf.trace_cap / 2;
+      bdf.trace_cap /= 2;
+      rtrace = bdf.trace + (size_t)bdf.trace_cap * TRACE_REC;
+    }
   }
   const double* ts = a.ts + (size_t)a.ts_stride * I;
   const double* dJ = a.dJydx + I * a.nt * NX;
@@ -1358,7 +1383,8 @@ __device__ __forceinline__ void simulate_backward(int inst, const BatchArgs& a, 
         team_sync();
         for (int k = tm.lane; k < NW; k += T) spill[k] = l64[k];
         team_sync();
-        const int r = recompute_interval<M>(a, tm, s, *lds_f, inst, c, bdf.sp.deadline);
+        const int r = recompute_interval<M>(a, tm, s, *lds_f, inst, c, bdf.sp.deadline, rtrace,
+                                            rtrace_cap, rtrace_n);
         team_sync();
         for (int k = tm.lane; k < NW; k += T) l64[k] = spill[k];
         team_sync();

@@ -207,7 +207,10 @@ int amx_run_sequential(const amx_solver_opts* opts, const amx_batch_in* in, amx_
  * half (device memory, `capacity` records of 8 doubles: event (1 step,
  * 2 nonlinear-solver failure, 3 error-test failure), t, h, order, error norm
  * or flag, cumulative Newton iterations, linear-solver setups, Jacobian
- * evaluations).  Applies to
+ * evaluations).  With adjoint checkpoints (ckpt_steps > 0) the backward
+ * records take the third quarter of the buffer and the forward steps
+ * recomputed from checkpoints the last quarter, interval by interval in the
+ * order of recomputation (counters from 0 in each interval).  Applies to
  * the launches that follow, from any thread (a process-wide setting; the
  * library serialises access to it). */
 int amx_set_trace(double* buffer, int instance, int capacity);

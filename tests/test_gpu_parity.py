@@ -447,3 +447,38 @@ def test_n200_adjoint_checkpoints(n200_model):
     assert (out["stats"][:, 0] > 3 * 32).all()
     _check_llh(n200_model.network, out, g)
     _close(out["sllh"], g["sllh_asa"], 1e-5, 1e-7, "sllh (adjoint, checkpoints every 32)")
+
+
+def test_checkpoint_recomputation_replays_forward_steps(models):
+    """The steps recomputed from the checkpoints in the backward sweep are the
+    forward pass's own: for one traced instance, every accepted step and every
+    failed attempt of each recomputed interval (event, time, step size, order,
+    error norm) equals the forward pass's record bit for bit, interval by
+    interval in the order of recomputation (the last interval first)."""
+    import torch
+    from amici_amd import sim
+    K, inst, cap = 32, 3, 1 << 16
+    g = _load("n40_conditions16")
+    lib = models["n40"].lib.lib
+    buf = torch.zeros((cap, 8), dtype=torch.float64, device="cuda")
+    assert lib.amx_set_trace(buf.data_ptr(), inst, cap) == 0
+    try:
+        out = _run(models, "n40", g, sim.SensitivityMethod.adjoint, ckpt_steps=K)
+    finally:
+        lib.amx_set_trace(None, 0, 0)
+    assert (out["status"] == 0).all(), out["status"]
+    tr = buf.cpu().numpy()
+    fwd = tr[: cap // 2]
+    rec = tr[cap // 2 + cap // 4:]
+    fwd, rec = fwd[fwd[:, 0] > 0], rec[rec[:, 0] > 0]
+    nst = int(out["stats"][inst, 0])
+    assert int((fwd[:, 0] == 1).sum()) == nst
+    # forward records grouped by checkpoint interval: interval c holds the
+    # accepted steps c K + 1 .. (c + 1) K and the failed attempts before them
+    step_no = np.cumsum(fwd[:, 0] == 1) + (fwd[:, 0] != 1)
+    interval = (step_no - 1) // K
+    nrec = nst // K  # intervals before the last one, which the forward pass kept
+    assert nrec >= 3
+    want = np.concatenate([fwd[interval == c] for c in range(nrec - 1, -1, -1)])
+    assert rec.shape == want.shape, (rec.shape, want.shape)
+    assert np.array_equal(rec[:, :5], want[:, :5])
