@@ -195,23 +195,39 @@ def _fmt_array(ctype: str, name: str, vals) -> str:
     return f"  static constexpr {ctype} {name}[{len(vals)}] = {{{body}}};\n"
 
 
-def emit_model_source(net: ReactionNetwork, team: int | None = None) -> str:
+def emit_model_source(net: ReactionNetwork, team: int | None = None,
+                      waves: int | None = None) -> str:
     """Model module source: traits plus the C ABI (amx_capi.inc)."""
-    return emit_model_traits(net, team) + '#include "amx_capi.inc"\n'
+    return emit_model_traits(net, team, waves) + '#include "amx_capi.inc"\n'
 
 
-def emit_model_traits(net: ReactionNetwork, team: int | None = None) -> str:
-    """The AmxModel traits struct (dimensions, team size, sparsity tables)."""
+WAVES_CHOICES = (1, 2, 4)
+
+
+def default_waves(nx: int) -> int:
+    """Register budget of the default module (Launch<M>::WAVES_PER_EU)."""
+    return 2 if nx <= 16 else 1
+
+
+def emit_model_traits(net: ReactionNetwork, team: int | None = None,
+                      waves: int | None = None) -> str:
+    """The AmxModel traits struct (dimensions, team size, sparsity tables).
+    ``waves``: register budget of the kernels as resident waves per SIMD (1:
+    512 registers per lane, 2: 256, 4: 128); None keeps the default (2 up to
+    16 states, else 1)."""
     t = model_tables(net)
     team = team or default_team(net.nx)
     if team < net.nx or team not in (64, 128, 256):
         raise ValueError(f"team size {team} does not fit nx={net.nx} (one lane per state; "
                          "a team is one 64-lane wavefront or a workgroup of 128 / 256 lanes)")
+    if waves is not None and waves not in WAVES_CHOICES:
+        raise ValueError(f"waves per SIMD must be one of {WAVES_CHOICES}")
     lines = [
         f"// Generated model module for network '{net.name}' (amici_amd.codegen).\n",
         "// Sparsity tables of xdot = S w, J, J^T v, df/dp and y = C x; see\n",
         "// amici_amd/codegen.py:model_tables for the meaning of every table.\n",
         f"#define AMX_TEAM_LANES {team if team > 64 else 64}\n",
+        *([f"#define AMX_WAVES_PER_EU {waves}\n"] if waves is not None else []),
         '#include "amx_massaction.h"\n\n',
         "struct AmxModel {\n",
         f"  static constexpr int NX = {net.nx}, NR = {net.nr}, NP = {net.np}, NK = {net.nk},"
@@ -234,9 +250,17 @@ def emit_model_traits(net: ReactionNetwork, team: int | None = None) -> str:
     return "".join(lines)
 
 
-def model_manifest(net: ReactionNetwork, team: int) -> dict:
+def model_manifest(net: ReactionNetwork, team: int, waves: int | None = None) -> dict:
     return dict(name=net.name, nx=net.nx, nr=net.nr, np=net.np, nk=net.nk, ny=net.ny,
-                team=team, noise=int(net.noise))
+                team=team, waves=waves, noise=int(net.noise))
+
+
+def variant_name(net: ReactionNetwork, team: int | None = None, waves: int | None = None) -> str:
+    """Module name of a launch-shape variant: the network's name for the
+    default shape, with _t<team> / _w<waves> for the others."""
+    team = team or default_team(net.nx)
+    name = net.name if team == default_team(net.nx) else f"{net.name}_t{team}"
+    return name if waves in (None, default_waves(net.nx)) else f"{name}_w{waves}"
 
 
 def _source_digest(src: str) -> str:
@@ -252,15 +276,19 @@ def model_dir(name: str) -> Path:
 
 
 def compile_model(net: ReactionNetwork, team: int | None = None, force: bool = False,
-                  name: str | None = None) -> Path:
+                  name: str | None = None, waves: int | None = None) -> Path:
     """Generate and compile the model module; returns the path of the .so.
+    ``team`` and ``waves`` select a launch-shape variant (lanes per instance,
+    register budget; see emit_model_traits).
 
     Reuses an existing library when its recorded source digest matches
     (unless ``force``).
     """
     team = team or default_team(net.nx)
-    name = name or (net.name if team == default_team(net.nx) else f"{net.name}_t{team}")
-    src = emit_model_source(net, team)
+    if waves == default_waves(net.nx):
+        waves = None  # the default module
+    name = name or variant_name(net, team, waves)
+    src = emit_model_source(net, team, waves)
     d = model_dir(name)
     d.mkdir(parents=True, exist_ok=True)
     so = d / f"libamx_{name}.so"
@@ -277,7 +305,7 @@ def compile_model(net: ReactionNetwork, team: int | None = None, force: bool = F
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed for model {name}:\n{' '.join(cmd)}\n{res.stderr[-8000:]}")
     shutil.move(str(tmp), str(so))
-    man = model_manifest(net, team)
+    man = model_manifest(net, team, waves)
     man["digest"] = digest
     man_path.write_text(json.dumps(man, indent=1))
     return so

@@ -148,8 +148,14 @@ struct Launch {
   // one instance per SIMD keep two resident per SIMD (4096 Robertson
   // conditions with forward sensitivities: 110 ms per batch at two, 171 ms at
   // one, 135 ms at four waves, where spills dominate); larger models keep the
-  // full 512 registers of one wave per SIMD.
+  // full 512 registers of one wave per SIMD.  A module generated with
+  // AMX_WAVES_PER_EU (a launch-shape variant, chosen by measurement) uses
+  // that budget instead.
+#ifdef AMX_WAVES_PER_EU
+  static constexpr int WAVES_PER_EU = AMX_WAVES_PER_EU;
+#else
   static constexpr int WAVES_PER_EU = M::NX <= 16 ? 2 : 1;
+#endif
 };
 
 // ---------------------------------------------------------------------------

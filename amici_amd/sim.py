@@ -49,11 +49,16 @@ class Model:
     """A compiled mass-action model (network + HIP model module)."""
 
     def __init__(self, network: ReactionNetwork, library: str | None = None,
-                 team: int | None = None):
+                 team: int | None = None, waves: int | None = None):
+        """``team`` / ``waves``: a launch-shape variant of the model module
+        (lanes per instance, register budget in waves per SIMD); the defaults
+        suit most batches, and ``amici_amd.autotune.tune`` picks them by
+        measurement."""
         from . import codegen
 
         self.network = network
-        path = library or codegen.compile_model(network, team)
+        self.team, self.waves = team, waves
+        path = library or codegen.compile_model(network, team, waves=waves)
         self.lib = _lib.ModelLib(path)
         if (self.lib.nx, self.lib.np, self.lib.nk, self.lib.ny) != (
                 network.nx, network.np, network.nk, network.ny):
