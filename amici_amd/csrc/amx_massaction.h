@@ -1230,7 +1230,7 @@ __device__ __forceinline__ void simulate_forward(int inst, const BatchArgs& a, c
 // With a step trace (trace != nullptr) the recomputed steps are appended to
 // it from record trace_n on.
 template <class M>
-__device__ __attribute__((noinline)) int recompute_interval(const BatchArgs& a, const Team<M::T>& tm,
+__device__ __forceinline__ int recompute_interval(const BatchArgs& a, const Team<M::T>& tm,
                                                            Smem<M>& s, BdfLds<M::NX, 0>& lds,
                                                            int inst, int c,
                                                            unsigned long long deadline,
@@ -1374,12 +1374,12 @@ __device__ __forceinline__ void simulate_backward(int inst, const BatchArgs& a, 
   const int mxB = a.spB.maxsteps;
   // start time of checkpoint interval k (record k's time; t0 for k = 0)
   const double* ckr = CK ? a.ck_rec + I * (size_t)a.ck_max * BDF<PB>::CK_LEN : nullptr;
-  auto ck_t = [&](int k) -> double {
+  auto ck_t = [&](int k) __attribute__((always_inline)) -> double {
     return k > 0 ? ckr[(size_t)k * BDF<PB>::CK_LEN + LMAX * NX + 7 + 6 + 6 + 15] : t0;
   };
   // moves to the interval before the current one while the backward problem
   // is at or before the current one's start, recomputing it
-  auto enter = [&]() -> int {
+  auto enter = [&]() __attribute__((always_inline)) -> int {
     if constexpr (CK) {
       for (int g = 0; g < n && c > 0 && bdf.tn <= ck_t(c); ++g) {
         --c;
@@ -1403,7 +1403,7 @@ __device__ __forceinline__ void simulate_backward(int inst, const BatchArgs& a, 
     return AMX_SUCCESS;
   };
   // CVode towards tnext, across checkpoint intervals (CK)
-  auto advance_to = [&](double tnext) -> int {
+  auto advance_to = [&](double tnext) __attribute__((always_inline)) -> int {
     for (int g = 0; g <= n; ++g) {
       int r = enter();
       if (r != AMX_SUCCESS) return r;
@@ -1414,7 +1414,7 @@ __device__ __forceinline__ void simulate_backward(int inst, const BatchArgs& a, 
     }
     return AMX_ERR_FAILURE;
   };
-  auto reinit_to = [&](double tnext) -> int {
+  auto reinit_to = [&](double tnext) __attribute__((always_inline)) -> int {
     const int r = enter();
     if (r != AMX_SUCCESS) return r;
     const double tst = ck_t(c) > tnext ? ck_t(c) : tnext;
