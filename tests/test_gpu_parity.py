@@ -482,3 +482,41 @@ def test_checkpoint_recomputation_replays_forward_steps(models):
     want = np.concatenate([fwd[interval == c] for c in range(nrec - 1, -1, -1)])
     assert rec.shape == want.shape, (rec.shape, want.shape)
     assert np.array_equal(rec[:, :5], want[:, :5])
+
+
+def test_checkpoints_with_preequilibration(open_model):
+    """Checkpoints every 16 steps after a pre-equilibration: the recomputed
+    intervals start from the steady state (record 0), and the adjoint's
+    steady-state term is added at t0 as without checkpoints; gradient against
+    the oracle."""
+    import torch
+    from amici_amd import sim
+    g = _load("n40open_preeq16")
+    sol = sim.Solver(**TIGHT, ckpt_steps=16)
+    sol.set_sensitivity_order(sim.SensitivityOrder.first)
+    sol.set_sensitivity_method(sim.SensitivityMethod.adjoint)
+    out = sim.simulate_batch(open_model, sol, g["theta"], g["kfix"], g["ts"], g["meas"], g["sigma"],
+                             kfix_preeq=g["kfix_preeq"])
+    torch.cuda.synchronize()
+    out = {k: (v.cpu().numpy() if hasattr(v, "cpu") else v) for k, v in out.items()
+           if k != "_keepalive"}
+    assert (out["status"] == 0).all(), out["status"]
+    assert (out["stats"][:, 0] > 3 * 16).all()
+    _check_llh(open_model.network, out, g)
+    _close(out["sllh"], g["sllh_asa"], 1e-5, 1e-7, "sllh (adjoint, pre-equilibrated, checkpoints)")
+
+
+def test_checkpoint_step_limit(models):
+    """With checkpoints max_traj still bounds the forward points of an
+    instance: a limit below the steps taken fails every instance with the
+    trajectory-buffer status (-1010) and NaN gradients, and a limit above
+    them changes nothing."""
+    from amici_amd import sim
+    g = _load("n40_conditions16")
+    small = _run(models, "n40", g, sim.SensitivityMethod.adjoint, ckpt_steps=32, max_traj=200)
+    assert (small["status"] == -1010).all(), small["status"]
+    assert np.isnan(small["sllh"]).all()
+    big = _run(models, "n40", g, sim.SensitivityMethod.adjoint, ckpt_steps=32, max_traj=50000)
+    ref = _run(models, "n40", g, sim.SensitivityMethod.adjoint, ckpt_steps=32)
+    assert (ref["status"] == 0).all()
+    assert np.array_equal(big["sllh"], ref["sllh"]) and np.array_equal(big["llh"], ref["llh"])
