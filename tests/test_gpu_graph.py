@@ -21,18 +21,19 @@ def _np(out):
     return {k: v.cpu().numpy() for k, v in out.items() if v is not None and k != "_keepalive"}
 
 
-@pytest.mark.parametrize("name,fixture,meth", [("robertson", "robertson_batch64", 2),
-                                               ("robertson", "robertson_batch64", 1),
-                                               ("n40", "n40_conditions16", 2),
-                                               ("n40", "n40_conditions16", 0)])
-def test_graph_replays_equal_eager(gpu, name, fixture, meth):
+@pytest.mark.parametrize("name,fixture,meth,ckpt", [("robertson", "robertson_batch64", 2, 0),
+                                                    ("robertson", "robertson_batch64", 1, 0),
+                                                    ("n40", "n40_conditions16", 2, 0),
+                                                    ("n40", "n40_conditions16", 0, 0),
+                                                    ("n40", "n40_conditions16", 2, 32)])
+def test_graph_replays_equal_eager(gpu, name, fixture, meth, ckpt):
     import torch
     from amici_amd import sim
 
     g = dict(np.load(GOLD / f"{fixture}.npz"))
     net = networks.robertson() if name == "robertson" else networks.network_n40()
     model = sim.Model(net)
-    sol = sim.Solver(**BENCH)
+    sol = sim.Solver(**BENCH, ckpt_steps=ckpt)  # ckpt: adjoint checkpoints every 32 steps
     if meth:
         sol.set_sensitivity_order(sim.SensitivityOrder.first)
         sol.set_sensitivity_method(meth)

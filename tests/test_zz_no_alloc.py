@@ -92,3 +92,44 @@ def test_static_workspace_limit_refused(gpu):
     assert rc == -24
     torch.cuda.synchronize()
     assert (p["out"]["status"].cpu().numpy() == sim.AMICI_NOT_RUN).all()
+
+
+def test_static_workspace_with_checkpoints(gpu):
+    """With adjoint checkpoints every 32 steps the static array holds 64 N40
+    conditions at maxsteps 10000 (storing every step they would need 212 MB);
+    two calls in a row, 64 then 16 conditions, equal amx_run_batch bit for bit."""
+    import torch
+    from amici_amd import sim
+    g = dict(np.load(GOLD / "n40_conditions16.npz"))
+    g = {k: (np.concatenate([v] * 4) if k in ("theta", "kfix", "meas", "sigma") else v)
+         for k, v in g.items()}
+    model = sim.Model(networks.network_n40())
+    L = model.lib.lib
+    full = sim.Solver(**BENCH)
+    full.set_sensitivity_order(sim.SensitivityOrder.first)
+    full.set_sensitivity_method(sim.SensitivityMethod.adjoint)
+    sol = sim.Solver(**BENCH, ckpt_steps=32)
+    sol.set_sensitivity_order(sim.SensitivityOrder.first)
+    sol.set_sensitivity_method(sim.SensitivityMethod.adjoint)
+    stream = torch.cuda.current_stream()
+    cap = L.amx_static_workspace_bytes()
+    pf = _structs(model, full, g, 64, gpu)
+    assert model.lib.workspace_bytes(pf["opts"], 64, pf["nt"]) > cap
+    for nb in (64, 16):
+        ps = _structs(model, sol, g, nb, gpu)
+        need = model.lib.workspace_bytes(ps["opts"], nb, ps["nt"])
+        assert need <= cap, (need, cap)
+        rc = L.amx_run_batch_static(ctypes.byref(ps["opts"]), ctypes.byref(ps["bin"]),
+                                    ctypes.byref(ps["bout"]), ctypes.c_void_p(stream.cuda_stream))
+        assert rc == 0, rc
+        pa = _structs(model, sol, g, nb, gpu)
+        ws = torch.empty(need, dtype=torch.uint8, device=gpu)
+        rc = L.amx_run_batch(ctypes.byref(pa["opts"]), ctypes.byref(pa["bin"]),
+                             ctypes.byref(pa["bout"]), ws.data_ptr(), ws.numel(),
+                             ctypes.c_void_p(stream.cuda_stream))
+        assert rc == 0, rc
+        torch.cuda.synchronize()
+        hs, ha = _host(ps), _host(pa)
+        assert (ha["status"] == 0).all(), ha["status"]
+        for k in ha:
+            assert np.array_equal(hs[k], ha[k], equal_nan=True), (nb, k)
