@@ -1119,14 +1119,15 @@ __device__ __forceinline__ void simulate_forward(int inst, const BatchArgs& a, c
     // steady-state sensitivities by the implicit function theorem:
     // sx0 = -J(x_ss)^{-1} df/dp for the species not reset from kfix
     if (a.kfix_pre && status == AMX_SUCCESS) {
-      LURows<NX, T> lu;
-      if (!factor_jacobian<M, false>(tm, s, pb.ev, xss, lu, lds)) {
+      JacLU<M> jl;
+      jl.mat = mat;
+      if (!jl.template factor<false>(tm, s, pb.ev, xss, lds)) {
         status = AMX_SINGULAR_JACOBIAN;
       } else {
         pb.sens_prepare(xss);
         for (int is = 0; is < pb.ns; ++is) {
           const double fp = pb.sens_f(0.0, is);
-          const double v = lu.solve(tm.lane < NX ? -fp : 0.0, tm, lds);
+          const double v = jl.solve(tm.lane < NX ? -fp : 0.0, tm, lds);
           if (tm.lane < NX) pb.S(0, is) = (M::X0K[tm.lane] >= 0) ? 0.0 : v;
         }
       }
@@ -1595,10 +1596,6 @@ __device__ __forceinline__ void note_failure(const BatchArgs& a, const TM& tm, i
 template <class M, int MODE>
 __device__ __forceinline__ void forward_kernel_body(BatchArgs a) {
   constexpr int T = M::T;
-  // forward sensitivities of models with more than 64 states are not
-  // provided (the C ABI rejects them before launch)
-  if constexpr (MODE == 1 && big_model<M>()) return;
-  else {
   __shared__ Smem<M> smem[Launch<M>::TEAMS];
   __shared__ BdfLds<M::NX, 0> blds[Launch<M>::TEAMS];
   Team<T> tm;
@@ -1610,7 +1607,6 @@ __device__ __forceinline__ void forward_kernel_body(BatchArgs a) {
     if (skip_failfast<M>(a, tm, inst)) continue;
     simulate_forward<M, MODE>(inst, a, tm, s, lds);
     note_failure(a, tm, inst);
-  }
   }
 }
 

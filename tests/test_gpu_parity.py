@@ -213,11 +213,33 @@ def test_n200_adjoint(n200_model):
     _close(out["sllh"], g["sllh_asa"], 1e-5, 1e-7, "sllh (adjoint)")
 
 
-def test_n200_rejects_forward_sensitivities(n200_model):
+def test_n200_forward_sensitivities(n200_model):
+    """Staggered forward sensitivities on the workgroup engine (each
+    sensitivity's Newton correction solved with the global-memory LU) for a
+    parameter list of 6 of the 600 parameters: sllh against the oracle's
+    gradient (its adjoint, which agrees with its forward sensitivities to the
+    same bar on the other fixtures), states against the oracle."""
+    import torch
     from amici_amd import sim
     g = _load("n200_conditions4")
-    with pytest.raises(RuntimeError, match="-22"):
-        _run({"m": n200_model}, "m", g, sim.SensitivityMethod.forward)
+    plist = [0, 7, 150, 299, 451, 599]
+    sol = sim.Solver(**TIGHT)
+    sol.set_sensitivity_order(sim.SensitivityOrder.first)
+    sol.set_sensitivity_method(sim.SensitivityMethod.forward)
+    out = sim.simulate_batch(n200_model, sol, g["theta"], g["kfix"], g["ts"], g["meas"], g["sigma"],
+                             want_sx=True, plist=plist)
+    torch.cuda.synchronize()
+    out = {k: (v.cpu().numpy() if hasattr(v, "cpu") else v) for k, v in out.items()
+           if k != "_keepalive"}
+    _dump("fsa_n200", out)
+    assert (out["status"] == 0).all(), out["status"]
+    assert out["sllh"].shape == (4, len(plist)) and out["sx"].shape == (4, 40, len(plist), 200)
+    _close(out["x"], g["x"], Y_RTOL, Y_FLOOR, "x")
+    _check_llh(n200_model.network, out, g)
+    ref = g["sllh_asa"][:, plist]
+    floor = 1e-7 * np.abs(g["sllh_asa"]).max(axis=1, keepdims=True)
+    err = np.abs(out["sllh"] - ref) / (1e-5 * np.abs(ref) + floor)
+    assert err.max() <= 1.0, (err.max(), out["sllh"], ref)
 
 
 @pytest.mark.parametrize("meth", [0, 2])
@@ -520,3 +542,31 @@ def test_checkpoint_step_limit(models):
     ref = _run(models, "n40", g, sim.SensitivityMethod.adjoint, ckpt_steps=32)
     assert (ref["status"] == 0).all()
     assert np.array_equal(big["sllh"], ref["sllh"]) and np.array_equal(big["llh"], ref["llh"])
+
+
+def test_n200_preequilibration_forward_sensitivities(gpu):
+    """Forward sensitivities after a pre-equilibration on the workgroup
+    engine: the steady state's implicit-function sensitivities come from the
+    global-memory LU of the Jacobian; a parameter list of 5 parameters against
+    the oracle's gradient."""
+    import torch
+    from amici_amd import sim
+    model = sim.Model(networks.network_n200_open())
+    g = _load("n200open_preeq4")
+    plist = [0, 3, 200, 512, model.np - 1]
+    sol = sim.Solver(**TIGHT)
+    sol.set_relative_tolerance_steady_state(1e-12)
+    sol.set_absolute_tolerance_steady_state(1e-16)
+    sol.set_sensitivity_order(sim.SensitivityOrder.first)
+    sol.set_sensitivity_method(sim.SensitivityMethod.forward)
+    out = sim.simulate_batch(model, sol, g["theta"], g["kfix"], g["ts"], g["meas"], g["sigma"],
+                             kfix_preeq=g["kfix_preeq"], plist=plist)
+    torch.cuda.synchronize()
+    out = {k: (v.cpu().numpy() if hasattr(v, "cpu") else v) for k, v in out.items()
+           if k != "_keepalive"}
+    assert (out["status"] == 0).all(), out["status"]
+    _close(out["x"], g["x"], Y_RTOL, Y_FLOOR, "x")
+    ref = g["sllh_asa"][:, plist]
+    floor = 1e-7 * np.abs(g["sllh_asa"]).max(axis=1, keepdims=True)
+    err = np.abs(out["sllh"] - ref) / (1e-5 * np.abs(ref) + floor)
+    assert err.max() <= 1.0, (err.max(), out["sllh"], ref)
