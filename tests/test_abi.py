@@ -79,9 +79,6 @@ def test_invalid_arguments_rejected_on_host(lib):
     outs = _lib.BatchOut(None, None, None, p, p, p, None)
     st = _lib.make_opts()
     assert fn(ctypes.byref(st), ctypes.byref(ins), ctypes.byref(outs), p, 8, None) == -23
-    if net.nx > 64:
-        # workgroup-team models: states and adjoint gradients only
-        assert fn(ctypes.byref(o), ctypes.byref(ins), ctypes.byref(outs), p, 1 << 30, None) == -22
     bad = _lib.make_opts(rtol=-1.0)
     assert fn(ctypes.byref(bad), ctypes.byref(ins), ctypes.byref(outs), p, 8, None) == -22
     bad = _lib.make_opts(rtol=math.nan)
