@@ -848,14 +848,34 @@ __device__ __forceinline__ ObjOut objective_at(const Team<M::T>& tm, Smem<M>& s,
     const double mv = meas ? meas[k] : NAN;
     if (!isnan(mv)) {
       const double sg = sigma[k];
-      if (M::NOISE == 1) {
-        const double r = (log(yk) - log(mv)) / sg;
-        nllh += 0.5 * log(2.0 * M_PI * sg * sg * mv * mv) + 0.5 * r * r;
-        g = r / (sg * yk);
+      // Jy and dJy/dy of the noise model (importers/utils.py:434-461): r is
+      // the difference of y and m on the model's scale (linear, ln or log10)
+      // divided by sigma, and dr/dy = 1 / dd with dd = sigma, sigma y or
+      // sigma y ln 10
+      constexpr int NZ = M::NOISE;
+      constexpr bool LAP = NZ >= 3;
+      constexpr int SC = NZ % 3;  // 0 linear, 1 ln, 2 log10 (NOISE ids 0..5)
+      double r, dd, jc;
+      if (SC == 0) {
+        r = (yk - mv) / sg;
+        dd = sg;
+        jc = LAP ? log(2.0 * sg) : 0.5 * log(2.0 * M_PI * sg * sg);
+      } else if (SC == 1) {
+        r = (log(yk) - log(mv)) / sg;
+        dd = sg * yk;
+        jc = LAP ? log(2.0 * sg * mv) : 0.5 * log(2.0 * M_PI * sg * sg * mv * mv);
       } else {
-        const double r = (yk - mv) / sg;
-        nllh += 0.5 * log(2.0 * M_PI * sg * sg) + 0.5 * r * r;
-        g = r / sg;
+        r = (log10(yk) - log10(mv)) / sg;
+        dd = sg * yk * M_LN10;
+        jc = LAP ? log(2.0 * sg * mv * M_LN10)
+                 : 0.5 * log(2.0 * M_PI * sg * sg * mv * mv * M_LN10 * M_LN10);
+      }
+      if (LAP) {
+        nllh += jc + fabs(r);
+        g = (r > 0.0 ? 1.0 : (r < 0.0 ? -1.0 : 0.0)) / dd;
+      } else {
+        nllh += jc + 0.5 * r * r;
+        g = r / dd;
       }
     }
     s.w[k] = g;

@@ -26,8 +26,31 @@ from typing import Sequence
 
 import numpy as np
 
+# Noise models of the observables (the continuous ones of AMICI's
+# noise_distribution, python/sdist/amici/importers/utils.py:434-461): the
+# distribution of the measurement m around the observable y with scale sigma,
+# on the linear, natural-log or log10 scale.
 NOISE_NORMAL = 0
 NOISE_LOGNORMAL = 1
+NOISE_LOG10NORMAL = 2
+NOISE_LAPLACE = 3
+NOISE_LOGLAPLACE = 4
+NOISE_LOG10LAPLACE = 5
+NOISE_NAMES = {"normal": NOISE_NORMAL, "lin-normal": NOISE_NORMAL, "log-normal": NOISE_LOGNORMAL,
+               "lognormal": NOISE_LOGNORMAL, "log10-normal": NOISE_LOG10NORMAL,
+               "laplace": NOISE_LAPLACE, "lin-laplace": NOISE_LAPLACE,
+               "log-laplace": NOISE_LOGLAPLACE, "log10-laplace": NOISE_LOG10LAPLACE}
+
+
+def noise_scale(noise: int) -> int:
+    """Observable scaling of a noise model: 0 linear, 1 natural log, 2 log10
+    (ObservableScaling, used by the residuals)."""
+    return {NOISE_NORMAL: 0, NOISE_LAPLACE: 0, NOISE_LOGNORMAL: 1, NOISE_LOGLAPLACE: 1,
+            NOISE_LOG10NORMAL: 2, NOISE_LOG10LAPLACE: 2}[int(noise)]
+
+
+def noise_is_laplace(noise: int) -> bool:
+    return int(noise) in (NOISE_LAPLACE, NOISE_LOGLAPLACE, NOISE_LOG10LAPLACE)
 
 
 @dataclasses.dataclass
@@ -353,6 +376,13 @@ def linear_chain(n: int = 3, seed: int = 11) -> ReactionNetwork:
 # ---------------------------------------------------------------------------
 # condition sets of the benchmark configurations
 
+def with_noise(net: ReactionNetwork, noise: int) -> ReactionNetwork:
+    """The same network with another noise model (its own model module)."""
+    if int(noise) == int(net.noise):
+        return net
+    return dataclasses.replace(net, name=f"{net.name}_noise{int(noise)}", noise=int(noise))
+
+
 def robertson_times() -> np.ndarray:
     """40 log-spaced output times 1e-4 .. 4e5."""
     return np.logspace(-4.0, math.log10(4e5), 40)
@@ -372,11 +402,15 @@ def input_conditions(net: ReactionNetwork, n: int, seed: int) -> np.ndarray:
 
 
 def add_noise(y: np.ndarray, noise: int, sigma: float, seed: int) -> np.ndarray:
-    """Synthetic measurements: additive Gaussian or multiplicative log-normal noise."""
+    """Synthetic measurements drawn from the noise model around y: Gaussian
+    or Laplace on the linear, natural-log or log10 scale."""
     rng = np.random.default_rng(seed)
-    e = rng.standard_normal(y.shape)
-    if noise == NOISE_LOGNORMAL:
+    e = rng.laplace(size=y.shape) if noise_is_laplace(noise) else rng.standard_normal(y.shape)
+    sc = noise_scale(noise)
+    if sc == 1:
         return y * np.exp(sigma * e)
+    if sc == 2:
+        return y * 10.0 ** (sigma * e)
     return y + sigma * e
 
 

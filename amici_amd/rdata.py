@@ -19,9 +19,11 @@ from __future__ import annotations
 
 import enum
 
+import math
+
 import torch
 
-from .networks import NOISE_LOGNORMAL
+from .networks import noise_scale
 
 
 class RDataReporting(enum.IntEnum):
@@ -63,11 +65,15 @@ def reported_fields(mode: RDataReporting, forward: bool, sensi: bool) -> set:
 
 
 def residuals(y: torch.Tensor, meas: torch.Tensor, sigma: torch.Tensor, noise: int) -> torch.Tensor:
-    """[nb, nt, ny]: (m - y) / sigma, on natural-log scale for log-normal noise
-    (fres, src/rdata.cpp:985-996); 0 where the measurement is missing."""
+    """[nb, nt, ny]: (m - y) / sigma on the noise model's observable scale
+    (linear, natural log or log10; fres, src/rdata.cpp:985-996); 0 where the
+    measurement is missing."""
     obs = ~torch.isnan(meas)
-    if noise == NOISE_LOGNORMAL:
+    sc = noise_scale(noise)
+    if sc == 1:
         r = (torch.log(meas) - torch.log(y)) / sigma
+    elif sc == 2:
+        r = (torch.log10(meas) - torch.log10(y)) / sigma
     else:
         r = (meas - y) / sigma
     return torch.where(obs, r, torch.zeros_like(r))
@@ -75,12 +81,16 @@ def residuals(y: torch.Tensor, meas: torch.Tensor, sigma: torch.Tensor, noise: i
 
 def residual_sensitivities(y: torch.Tensor, sy: torch.Tensor, meas: torch.Tensor,
                            sigma: torch.Tensor, noise: int) -> torch.Tensor:
-    """[nb, nt, ny, np] from sy [nb, nt, np, ny]: -sy / sigma, or -sy / (y sigma)
-    for log-normal noise (fsres with ssigma = 0, src/rdata.cpp:1045-1062)."""
+    """[nb, nt, ny, np] from sy [nb, nt, np, ny]: -sy / sigma, -sy / (y sigma)
+    on the natural-log scale, -sy / (y ln 10 sigma) on the log10 scale (fsres
+    with ssigma = 0, src/rdata.cpp:1045-1062)."""
     syt = sy.transpose(-1, -2)                       # [nb, nt, ny, np]
     scale = 1.0 / sigma
-    if noise == NOISE_LOGNORMAL:
+    sc = noise_scale(noise)
+    if sc == 1:
         scale = scale / y
+    elif sc == 2:
+        scale = scale / (y * math.log(10.0))
     s = -syt * scale.unsqueeze(-1)
     obs = ~torch.isnan(meas)
     return torch.where(obs.unsqueeze(-1), s, torch.zeros_like(s))

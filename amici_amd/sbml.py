@@ -34,7 +34,7 @@ from typing import Mapping, Sequence
 import numpy as np
 import sympy as sp
 
-from .networks import NOISE_LOGNORMAL, NOISE_NORMAL, Reaction, ReactionNetwork
+from .networks import NOISE_NAMES, Reaction, ReactionNetwork
 
 SBML_NS = "http://www.sbml.org/sbml/level3/version1/core"
 MATHML_NS = "http://www.w3.org/1998/Math/MathML"
@@ -130,7 +130,9 @@ def read_sbml(source, observables: Mapping[str, str] | None = None,
     ``observables`` maps observable ids to linear expressions in the species
     ids; ``fixed_parameters`` names the parameters that are per-condition
     inputs (they may only appear as species initial assignments); ``noise`` is
-    ``"normal"`` or ``"lognormal"``.
+    one of AMICI's continuous noise distributions (``"normal"``,
+    ``"log-normal"``, ``"log10-normal"``, ``"laplace"``, ``"log-laplace"``,
+    ``"log10-laplace"``; ``"lognormal"`` is accepted for ``"log-normal"``).
     """
     text = Path(source).read_text() if isinstance(source, Path) or (
         isinstance(source, str) and not source.lstrip().startswith("<")) else source
@@ -277,9 +279,16 @@ def read_sbml(source, observables: Mapping[str, str] | None = None,
         k_nominal=np.asarray([params[f] for f in fixed]),
         x0_const=np.where(x0_fixed >= 0, 0.0, np.asarray(x0)), x0_fixed=x0_fixed,
         observable_matrix=C,
-        noise=NOISE_LOGNORMAL if noise == "lognormal" else NOISE_NORMAL)
+        noise=_noise_id(noise))
     net.validate()
     return net
+
+
+def _noise_id(noise: str) -> int:
+    if noise not in NOISE_NAMES:
+        raise SbmlError(f"noise distribution {noise!r} is not supported "
+                        f"(one of {sorted(NOISE_NAMES)})")
+    return NOISE_NAMES[noise]
 
 
 def _ci(name):

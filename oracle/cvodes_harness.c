@@ -9,8 +9,8 @@
  *     (src/model_ode.cpp:426-450; J and df/dp refreshed at ip == 0)
  *   - adjoint RHS xBdot = -J^T xB, qBdot = -(df/dp)^T xB  -> Model_ODE::fxBdot /
  *     fqBdot / fJB (src/model_ode.cpp:331-374)
- *   - Gaussian / log-normal Jy and dJy/dy                  -> noise formulas
- *     (python/sdist/amici/importers/utils.py:434-440)
+ *   - Jy and dJy/dy of the continuous noise models         -> noise formulas
+ *     (python/sdist/amici/importers/utils.py:434-461)
  *   - forward output loop with tstop = tout               -> EventHandlingSimulator::run
  *     (src/forwardproblem.cpp:53-151), is_next_t_too_close (:26-38),
  *     Solver::run (src/solver.cpp:141-160), apply_max_num_steps (:107-119)
@@ -127,19 +127,37 @@ void amx_oracle_dxdotdp(const amx_oracle_model* m, const double* p,
     }
 }
 
+/* The cost functions of AMICI's continuous noise distributions
+ * (python/sdist/amici/importers/utils.py:434-461): noise 0 normal, 1 log-normal,
+ * 2 log10-normal, 3 laplace, 4 log-laplace, 5 log10-laplace.  u(v) is the
+ * model's scale (v, ln v or log10 v) and lnb its log base factor. */
+static double noise_u(int sc, double v) { return sc == 0 ? v : (sc == 1 ? log(v) : log10(v)); }
+
 double amx_oracle_Jy(int noise, double y, double sigma, double meas) {
-    const double two_pi = 2.0 * M_PI;
+    if (noise == 0) {
+        const double r = (y - meas) / sigma;
+        return 0.5 * log(2.0 * M_PI * sigma * sigma) + 0.5 * r * r;
+    }
     if (noise == 1) {
         const double r = (log(y) - log(meas)) / sigma;
-        return 0.5 * log(two_pi * sigma * sigma * meas * meas) + 0.5 * r * r;
+        return 0.5 * log(2.0 * M_PI * sigma * sigma * meas * meas) + 0.5 * r * r;
     }
-    const double r = (y - meas) / sigma;
-    return 0.5 * log(two_pi * sigma * sigma) + 0.5 * r * r;
+    const int sc = noise % 3;
+    const double r = (noise_u(sc, y) - noise_u(sc, meas)) / sigma;
+    /* the measurement's density carries the Jacobian of the scale, m ln b */
+    const double jac = sc == 0 ? 1.0 : (sc == 1 ? meas : meas * log(10.0));
+    if (noise >= 3) return log(2.0 * sigma * jac) + fabs(r);
+    return 0.5 * log(2.0 * M_PI * sigma * sigma * jac * jac) + 0.5 * r * r;
 }
 
 double amx_oracle_dJydy(int noise, double y, double sigma, double meas) {
+    if (noise == 0) return (y - meas) / (sigma * sigma);
     if (noise == 1) return (log(y) - log(meas)) / (sigma * sigma * y);
-    return (y - meas) / (sigma * sigma);
+    const int sc = noise % 3;
+    const double r = (noise_u(sc, y) - noise_u(sc, meas)) / sigma;
+    const double drdy = 1.0 / (sigma * (sc == 0 ? 1.0 : (sc == 1 ? y : y * log(10.0))));
+    if (noise >= 3) return (r > 0.0 ? 1.0 : (r < 0.0 ? -1.0 : 0.0)) * drdy;
+    return r * drdy;
 }
 
 /* ---------------- CVODES callbacks ---------------- */

@@ -247,6 +247,36 @@ def robertson_sx0() -> None:
                         sllh_asa=a["sllh"], sx=f["sx"])
 
 
+def robertson_noise() -> None:
+    """16 Robertson conditions (the first parameter sets of
+    robertson_batch64) with each of the noise models log10-normal, Laplace,
+    log-Laplace and log10-Laplace: measurements drawn from the model around the
+    oracle's observables (sigma 0.1, seed 17), then states, llh and the
+    forward and adjoint gradients at rtol 1e-10.  On the log scales the points
+    where the observable is below 1e-4 are missing: there (species A decays to
+    1e-10) the absolute tolerance 1e-14 leaves a relative error of the
+    observable that the log scale turns into gradient errors far above the
+    parity bar, for CVODES's forward and adjoint gradients alike."""
+    g = np.load(OUT / "robertson_batch64.npz")
+    nb = 16
+    base = networks.robertson()
+    theta, kfix, ts = g["theta"][:nb], g["kfix"][:nb], g["ts"]
+    sigma = np.full((nb, len(ts), base.ny), 0.1)
+    y = run(base, theta, kfix, ts, None, sigma, 0, TIGHT)["y"]
+    for noise in (networks.NOISE_LOG10NORMAL, networks.NOISE_LAPLACE, networks.NOISE_LOGLAPLACE,
+                  networks.NOISE_LOG10LAPLACE):
+        net = networks.with_noise(base, noise)
+        meas = networks.add_noise(y, noise, 0.1, seed=17)
+        if networks.noise_scale(noise):
+            meas = np.where(y < 1e-4, np.nan, meas)
+        args = (net, theta, kfix, ts, meas, sigma)
+        f = run(*args, 1, TIGHT)
+        a = run(*args, 2, TIGHT)
+        np.savez_compressed(OUT / f"robertson_noise{noise}.npz", noise=noise, theta=theta,
+                            kfix=kfix, ts=ts, meas=meas, sigma=sigma, x=a["x"], y=a["y"],
+                            llh=a["llh"], sllh_fsa=f["sllh"], sllh_asa=a["sllh"])
+
+
 def n200() -> None:
     """4 conditions of the 200-species network: states, observables, llh and
     the adjoint gradient at rtol 1e-10 / atol 1e-14, plus the rtol-1e-8
@@ -319,6 +349,8 @@ if __name__ == "__main__":
         robertson_mix()
     elif sys.argv[1:] == ["step_limit"]:
         step_limit()
+    elif sys.argv[1:] == ["robertson_noise"]:
+        robertson_noise()
     elif sys.argv[1:] == ["n200"]:
         n200()
     elif sys.argv[1:] == ["n200_preeq"]:

@@ -48,10 +48,11 @@ def _llh_tol(net, g):
     """Bound on |llh error| implied by the observable tolerance: the sum over
     measured points of |dJy/dy| times the allowed observable error."""
     y, m, s = g["y"], g["meas"], g["sigma"]
-    if int(net.noise) == networks.NOISE_LOGNORMAL:
-        dj = (np.log(y) - np.log(m)) / (s * s * y)
-    else:
-        dj = (y - m) / (s * s)
+    sc = networks.noise_scale(net.noise)
+    u = (lambda v: v) if sc == 0 else (np.log if sc == 1 else np.log10)
+    dd = s if sc == 0 else (s * y if sc == 1 else s * y * np.log(10.0))
+    r = (u(y) - u(m)) / s
+    dj = (np.sign(r) if networks.noise_is_laplace(net.noise) else r) / dd
     dy = Y_RTOL * np.abs(y) + Y_FLOOR * np.abs(y).reshape(y.shape[0], -1).max(1)[:, None, None]
     return np.nansum(np.abs(dj) * dy, axis=(1, 2)) + 1e-12 * np.abs(g["llh"])
 
@@ -570,3 +571,21 @@ def test_n200_preequilibration_forward_sensitivities(gpu):
     floor = 1e-7 * np.abs(g["sllh_asa"]).max(axis=1, keepdims=True)
     err = np.abs(out["sllh"] - ref) / (1e-5 * np.abs(ref) + floor)
     assert err.max() <= 1.0, (err.max(), out["sllh"], ref)
+
+
+@pytest.mark.parametrize("noise", [2, 3, 4, 5])
+def test_noise_models(gpu, noise):
+    """Robertson with the log10-normal, Laplace, log-Laplace and log10-Laplace
+    noise models (each its own model module): states, llh, forward and adjoint
+    gradients against the oracle."""
+    from amici_amd import sim
+    g = _load(f"robertson_noise{noise}")
+    net = networks.with_noise(networks.robertson(), noise)
+    models = {"m": sim.Model(net)}
+    for meth, key in ((sim.SensitivityMethod.forward, "sllh_fsa"),
+                      (sim.SensitivityMethod.adjoint, "sllh_asa")):
+        out = _run(models, "m", g, meth)
+        assert (out["status"] == 0).all(), out["status"]
+        _close(out["x"], g["x"], Y_RTOL, Y_FLOOR, "x")
+        _check_llh(net, out, g)
+        _close(out["sllh"], g[key], 1e-5, 1e-7, f"sllh ({key}, noise {noise})")

@@ -75,6 +75,33 @@ def test_noise_closed_forms():
             assert O.dJydy(noise, y, s, m) == pytest.approx(fd, rel=1e-7, abs=1e-9)
 
 
+@pytest.mark.parametrize("noise", [0, 1, 2, 3, 4, 5])
+def test_noise_models_are_log_densities(noise):
+    """Jy of every continuous noise model is minus the log density of the
+    measurement m given y and sigma (scipy.stats: Gaussian or Laplace on the
+    linear, ln or log10 scale, with the scale's Jacobian 1 / (m ln b)), it
+    integrates to one over m, and dJy/dy is its derivative."""
+    from scipy import integrate, stats
+    s = 0.3
+    sc = networks.noise_scale(noise)
+    lap = networks.noise_is_laplace(noise)
+    u = (lambda v: v) if sc == 0 else (np.log if sc == 1 else np.log10)
+    jac = (lambda m: 1.0) if sc == 0 else ((lambda m: m) if sc == 1 else (lambda m: m * math.log(10)))
+    dist = stats.laplace if lap else stats.norm
+    for y in (0.3, 1.7, 2.9):
+        for m in (0.2, 1.7, 3.5):
+            want = -(dist.logpdf(u(m), loc=u(y), scale=s) - math.log(jac(m)))
+            assert O.Jy(noise, y, s, m) == pytest.approx(want, rel=1e-12, abs=1e-12)
+            if m != y:
+                h = 1e-6
+                fd = (O.Jy(noise, y + h, s, m) - O.Jy(noise, y - h, s, m)) / (2 * h)
+                assert O.dJydy(noise, y, s, m) == pytest.approx(fd, rel=1e-6, abs=1e-8)
+        lo = -np.inf if sc == 0 else 0.0
+        total = sum(integrate.quad(lambda m: math.exp(-O.Jy(noise, y, s, m)), a, b, limit=200)[0]
+                    for a, b in ((lo, y), (y, np.inf)))
+        assert total == pytest.approx(1.0, rel=1e-7)
+
+
 def test_linear_states_and_fsa_closed_form():
     net = networks.linear_chain(3)
     ts = np.array([0.0, 0.1, 0.5, 1.0, 3.0, 10.0])

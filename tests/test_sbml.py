@@ -4,6 +4,8 @@ stoichiometry (python/sdist/amici/importers/sbml/__init__.py:1362-1428) and
 mass-action rate laws."""
 from __future__ import annotations
 
+import math
+
 import numpy as np
 import pytest
 
@@ -79,3 +81,34 @@ def test_reversible_law_splits_into_two_reactions():
 def test_outside_the_subset_is_rejected(size, law, obs):
     with pytest.raises(sbml.SbmlError):
         sbml.read_sbml(MODEL.format(size=size, law=law), observables=obs)
+
+
+def test_noise_distribution_names():
+    """AMICI's continuous noise-distribution names select the noise model;
+    others are refused."""
+    for name, want in (("normal", networks.NOISE_NORMAL), ("log-normal", networks.NOISE_LOGNORMAL),
+                       ("lognormal", networks.NOISE_LOGNORMAL),
+                       ("log10-normal", networks.NOISE_LOG10NORMAL),
+                       ("laplace", networks.NOISE_LAPLACE), ("log-laplace", networks.NOISE_LOGLAPLACE),
+                       ("log10-laplace", networks.NOISE_LOG10LAPLACE)):
+        net = sbml.read_sbml(DATA / "robertson.xml", observables={"yA": "A", "yC": "C"}, noise=name)
+        assert net.noise == want
+    with pytest.raises(sbml.SbmlError):
+        sbml.read_sbml(DATA / "robertson.xml", observables={"yA": "A"}, noise="binomial")
+
+
+def test_log10_residuals():
+    """Residuals on the log10 scale for the log10 noise models (fres,
+    rdata.cpp:985-996) and their sensitivities (fsres)."""
+    import torch
+    from amici_amd import rdata
+    y = torch.tensor([[[2.0, 0.5]]], dtype=torch.float64)
+    m = torch.tensor([[[3.0, float("nan")]]], dtype=torch.float64)
+    s = torch.full_like(y, 0.2)
+    for noise in (networks.NOISE_LOG10NORMAL, networks.NOISE_LOG10LAPLACE):
+        r = rdata.residuals(y, m, s, noise)
+        assert r[0, 0, 0].item() == pytest.approx((math.log10(3.0) - math.log10(2.0)) / 0.2)
+        assert r[0, 0, 1].item() == 0.0
+        sy = torch.tensor([[[[1.0, 1.0]]]], dtype=torch.float64)  # [nb, nt, np, ny]
+        sr = rdata.residual_sensitivities(y, sy, m, s, noise)
+        assert sr[0, 0, 0, 0].item() == pytest.approx(-1.0 / (2.0 * math.log(10.0) * 0.2))
