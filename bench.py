@@ -131,7 +131,7 @@ def main() -> int:
     ap.add_argument("--plain-sample", type=int, default=None,
                     help="conditions timed on the plain GPU BDF path (0 = skip; N40: 8)")
     ap.add_argument("--cpu-sample", type=int, default=None,
-                    help="conditions timed on the CPU baseline (0 = skip; N40: 128, N200: 16)")
+                    help="conditions timed on the CPU baseline (0 = skip; N40: 256, N200: 16)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES bytes 5 times (the known-size "
                          "kernel that scales the HBM counters) and exit")
