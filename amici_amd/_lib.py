@@ -9,7 +9,7 @@ import ctypes
 import math
 from pathlib import Path
 
-AMX_ABI_VERSION = 9
+AMX_ABI_VERSION = 10
 AMX_NSTATS = 18
 AMX_NDIAG = 9
 SENSI_NONE, SENSI_FORWARD, SENSI_ADJOINT = 0, 1, 2
@@ -48,7 +48,7 @@ class BatchIn(ctypes.Structure):
         ("meas", ctypes.c_void_p), ("sigma", ctypes.c_void_p),
         ("kfix_preeq", ctypes.c_void_p),
         ("nplist", ctypes.c_int), ("plist", ctypes.c_void_p), ("pscale", ctypes.c_void_p),
-        ("x0", ctypes.c_void_p), ("sx0", ctypes.c_void_p),
+        ("x0", ctypes.c_void_p), ("sx0", ctypes.c_void_p), ("t0", ctypes.c_void_p),
     ]
 
 

@@ -26,7 +26,7 @@ def _graph_for(model, solver, theta, kfix, ts, meas, sigma) -> sim.BatchGraph:
     key = (repr(sorted(vars(solver).items())), tuple(theta.shape), tuple(torch.as_tensor(ts).shape),
            tuple(torch.as_tensor(meas).shape), tuple(model.get_parameter_scale()),
            None if x0 is None else tuple(x0),
-           None if model._sx0 is None else model._sx0.tobytes())
+           None if model._sx0 is None else model._sx0.tobytes(), model._t0)
     cache = model.__dict__.setdefault("_nll_graphs", {})
     g = cache.get(key)
     if g is None:

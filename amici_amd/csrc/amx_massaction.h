@@ -39,6 +39,8 @@ struct BatchArgs {
   const int* pscale;       // [NP] ParameterScaling 0 none, 1 ln, 2 log10, or null (all log10)
   const double* x0;        // [nb][NX] initial states replacing the model's, or null
   const double* sx0;       // [nb][npl][NX] their sensitivities w.r.t. the scaled parameters, or null
+  const double* t0;        // [nb] start times (ExpData t_start), or null (all 0)
+  double* ts_rel;          // adjoint: [nb][nt] output times relative to the start time
   int* status;             // [nb]
   int* stats;              // [nb][AMX_NSTATS] or null
   int* diag;               // [nb][nt][AMX_NDIAG] per-output-time counters or null
@@ -76,6 +78,7 @@ struct BatchArgs {
   __device__ __forceinline__ int npl(int np) const { return plist ? nplist : np; }
   __device__ __forceinline__ int pidx(int i) const { return plist ? plist[i] : i; }
   __device__ __forceinline__ int scale(int j) const { return pscale ? pscale[j] : 2; }
+  __device__ __forceinline__ double tstart(int inst) const { return t0 ? t0[inst] : 0.0; }
 };
 
 // unscale_parameters (src/misc.cpp) for ParameterScaling none / ln / log10,
@@ -1114,6 +1117,11 @@ __device__ __forceinline__ void simulate_forward(int inst, const BatchArgs& a, c
   PB& pb = pb0;
   BDF<PB> bdf(pb, lds, a.sp);
   if (a.maxtime_ticks) bdf.sp.deadline = wall_clock64() + a.maxtime_ticks;
+  // The models are autonomous, so a start time t0 (ExpData t_start) only
+  // shifts the output grid: the integration runs in the time relative to t0,
+  // from 0, and an output time at or before t0 gives the initial state.  The
+  // adjoint passes read the relative output times the forward pass stores.
+  const double t0 = a.tstart(inst);
   bdf.init(0.0, x0, nullptr);
   if constexpr (CKM) bdf.save_state(pb.ck_rec, 0);  // checkpoint 0: the initial state
   if (a.trace && inst == a.trace_inst) {
@@ -1156,7 +1164,10 @@ __device__ __forceinline__ void simulate_forward(int inst, const BatchArgs& a, c
   double llh = 0.0;
   int it = 0;
   for (; it < a.nt && status == AMX_SUCCESS; ++it) {
-    const double tout = ts[it];
+    const double tout = fmax(ts[it] - t0, 0.0);
+    if constexpr (ADJ) {
+      if (tm.lane == 0) a.ts_rel[I * a.nt + it] = tout;
+    }
     if (tout > 0.0 && bdf.tn < tout && !next_t_too_close(bdf.tn, tout)) {
       // the step budget counts over the whole forward run; one used up
       // exactly at an output time is an exception in Solver::apply_max_num_steps
@@ -1281,7 +1292,7 @@ __device__ __forceinline__ int recompute_interval(const BatchArgs& a, const Team
     bdf.force_setup = true;
     pr.on_step(bdf.tn, bdf.Z(0), bdf.Z(1), bdf.hscale, bdf.q);
   }
-  const double* ts = a.ts + (size_t)a.ts_stride * I;
+  const double* ts = a.ts_rel + I * a.nt;  // as the forward pass saw them
   int r = AMX_ERR_FAILURE;  // the forward pass went on past this interval
   for (int j = jout; j < a.nt; ++j) {
     const double tout = ts[j];
@@ -1365,7 +1376,7 @@ __device__ __forceinline__ void simulate_backward(int inst, const BatchArgs& a, 
       rtrace = bdf.trace + (size_t)bdf.trace_cap * TRACE_REC;
     }
   }
-  const double* ts = a.ts + (size_t)a.ts_stride * I;
+  const double* ts = a.ts_rel + I * a.nt;  // relative to the start time (forward pass)
   const double* dJ = a.dJydx + I * a.nt * NX;
   const double t0 = 0.0;
   int it = a.nt - 1;

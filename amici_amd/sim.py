@@ -73,6 +73,7 @@ class Model:
         self._pscale = [_lib.PSCALE_LOG10] * network.np
         self._x0 = None
         self._sx0 = None
+        self._t0 = 0.0
 
     # dimensions
     @property
@@ -140,6 +141,14 @@ class Model:
         if x0.shape != (self.nx_rdata,):
             raise ValueError(f"expected {self.nx_rdata} initial states")
         self._x0 = x0.copy()
+
+    def set_t0(self, t0: float) -> None:
+        """Start time of the simulation (Model::set_t0; ExpData.t_start
+        overrides it per condition, src/edata.cpp:551)."""
+        self._t0 = float(t0)
+
+    def t0(self) -> float:
+        return self._t0
 
     def set_initial_state_sensitivities(self, sx0) -> None:
         """Initial state sensitivities with respect to the scaled parameters
@@ -381,7 +390,8 @@ class ExpData:
     applies (src/edata.cpp:438-490): free parameters (on their scale),
     parameter list ``plist``, parameter scales ``pscale``, initial states
     ``x0`` and their sensitivities ``sx0`` (nplist x nx, with respect to the
-    scaled parameters; empty = the model's)."""
+    scaled parameters; empty = the model's), and the start time ``t_start``
+    (None = the model's t0)."""
 
     def __init__(self, source: "Model | ReturnData | None" = None, sigma_y=None,
                  sigma_z: float = 0.0, seed: int = -1):
@@ -400,6 +410,7 @@ class ExpData:
             self.free_parameters = np.zeros(0)
             self.fixed_parameters_pre_equilibration = np.zeros(0)
             self.plist, self.pscale, self.x0, self.sx0 = [], [], np.zeros(0), np.zeros(0)
+            self.t_start = None
             nt = len(self.timepoints)
             sig = np.broadcast_to(np.asarray(1.0 if sigma_y is None else sigma_y, np.float64),
                                   (nt, self.ny)).copy()
@@ -417,6 +428,7 @@ class ExpData:
         self.free_parameters = np.zeros(0)
         self.fixed_parameters_pre_equilibration = np.zeros(0)
         self.plist, self.pscale, self.x0, self.sx0 = [], [], np.zeros(0), np.zeros(0)
+        self.t_start = None  # start time (None: the model's t0)
         self._meas = np.full((len(self.timepoints), self.ny), np.nan)
         self._sigma = np.ones((len(self.timepoints), self.ny))
         self.id = ""
@@ -527,7 +539,7 @@ def _ptr(t):
 
 def _prepare(model: Model, solver: Solver, theta, kfix, ts, meas, sigma, *, want_x, want_y,
              want_sx, want_stats, kfix_preeq, plist=None, pscale=None, x0=None,
-             sx0=None, failfast=False, copy_inputs=False) -> dict:
+             sx0=None, t0=None, failfast=False, copy_inputs=False) -> dict:
     """Device inputs, output tensors and the C structs of one batch call.
     With ``copy_inputs`` every input is a fresh device copy (for a graph that
     keeps pointing at them).  plist / pscale default to the model's settings
@@ -543,6 +555,13 @@ def _prepare(model: Model, solver: Solver, theta, kfix, ts, meas, sigma, *, want
         x0 = np.broadcast_to(model._x0, (np.shape(theta)[0], model.nx_rdata))
     if sx0 is None and model._sx0 is not None and model._sx0.shape[0] == npl:
         sx0 = np.broadcast_to(model._sx0, (np.shape(theta)[0], npl, model.nx_rdata))
+    if t0 is None:
+        t0 = model._t0
+    t0 = np.broadcast_to(np.asarray(t0.cpu() if isinstance(t0, torch.Tensor) else t0, np.float64),
+                         (np.shape(theta)[0],))
+    if not np.all(np.isfinite(t0)):
+        raise ValueError("start times must be finite")
+    t0 = None if not np.any(t0) else t0  # all 0: the kernels' default
 
     def dev_t(a):
         t = _as_dev(a, dev)
@@ -563,6 +582,7 @@ def _prepare(model: Model, solver: Solver, theta, kfix, ts, meas, sigma, *, want
     if x0_t is not None and x0_t.shape != (nb, model.nx_rdata):
         raise ValueError("x0 must be [nb, nx]")
     sx0_t = dev_t(sx0) if sx0 is not None else None
+    t0_t = dev_t(t0) if t0 is not None else None
     if sx0_t is not None and sx0_t.shape != (nb, npl, model.nx_rdata):
         raise ValueError("sx0 must be [nb, nplist, nx]")
     # the identity list and all-log10 scales are the kernels' defaults (NULL)
@@ -597,18 +617,20 @@ def _prepare(model: Model, solver: Solver, theta, kfix, ts, meas, sigma, *, want
     }
     bin_ = _lib.BatchIn(nb, nt, ts_stride, _ptr(theta_t), _ptr(kfix_t) if model.nk else None,
                         _ptr(ts_t), _ptr(meas_t), _ptr(sigma_t), _ptr(kpre_t),
-                        npl, _ptr(plist_t), _ptr(pscale_t), _ptr(x0_t), _ptr(sx0_t))
+                        npl, _ptr(plist_t), _ptr(pscale_t), _ptr(x0_t), _ptr(sx0_t),
+                        _ptr(t0_t))
     bout = _lib.BatchOut(*[_ptr(out[k]) for k in ("x", "y", "sx", "llh", "sllh", "status", "stats",
                                                   "diag")])
     inputs = dict(theta=theta_t, kfix=kfix_t, ts=ts_t, meas=meas_t, sigma=sigma_t,
-                  kfix_preeq=kpre_t, plist=plist_t, pscale=pscale_t, x0=x0_t, sx0=sx0_t)
+                  kfix_preeq=kpre_t, plist=plist_t, pscale=pscale_t, x0=x0_t, sx0=sx0_t,
+                  t0=t0_t)
     return dict(dev=dev, nb=nb, nt=nt, opts=opts, bin=bin_, bout=bout, out=out, inputs=inputs)
 
 
 def simulate_batch(model: Model, solver: Solver, theta, kfix, ts, meas=None, sigma=None, *,
                    want_x: bool = True, want_y: bool = True, want_sx: bool = False,
                    want_stats: bool = True, sequential: bool = False, stream=None,
-                   kfix_preeq=None, plist=None, pscale=None, x0=None, sx0=None,
+                   kfix_preeq=None, plist=None, pscale=None, x0=None, sx0=None, t0=None,
                    failfast: bool = False) -> dict:
     """Simulates ``nb`` instances in one batch; arrays may be numpy or torch.
 
@@ -620,7 +642,8 @@ def simulate_batch(model: Model, solver: Solver, theta, kfix, ts, meas=None, sig
     simulation.  ``plist`` (parameter indices; sllh [nb, len(plist)], sx [nb,
     nt, len(plist), nx]), ``pscale`` (one ParameterScaling per parameter) and
     ``x0`` [nb, nx] and ``sx0`` [nb, nplist, nx] (with respect to the scaled
-    parameters) default to the model's settings.  With ``failfast`` the
+    parameters) and ``t0`` (start times, scalar or [nb]) default to the
+    model's settings.  With ``failfast`` the
     kernels skip instances with a larger index than a failed one that have
     not started yet (status AMICI_NOT_RUN).
     """
@@ -628,7 +651,7 @@ def simulate_batch(model: Model, solver: Solver, theta, kfix, ts, meas=None, sig
 
     p = _prepare(model, solver, theta, kfix, ts, meas, sigma, want_x=want_x, want_y=want_y,
                  want_sx=want_sx, want_stats=want_stats, kfix_preeq=kfix_preeq, plist=plist,
-                 pscale=pscale, x0=x0, sx0=sx0, failfast=failfast)
+                 pscale=pscale, x0=x0, sx0=sx0, t0=t0, failfast=failfast)
     lib, dev, opts = model.lib, p["dev"], p["opts"]
     nbytes = lib.workspace_bytes(opts, 1 if sequential else p["nb"], p["nt"])
     st = stream if stream is not None else torch.cuda.current_stream(dev)
@@ -656,12 +679,12 @@ class BatchGraph:
     def __init__(self, model: Model, solver: Solver, theta, kfix, ts, meas=None, sigma=None, *,
                  want_x: bool = True, want_y: bool = True, want_sx: bool = False,
                  want_stats: bool = True, kfix_preeq=None, plist=None, pscale=None, x0=None,
-                 sx0=None):
+                 sx0=None, t0=None):
         import ctypes
 
         p = _prepare(model, solver, theta, kfix, ts, meas, sigma, want_x=want_x, want_y=want_y,
                      want_sx=want_sx, want_stats=want_stats, kfix_preeq=kfix_preeq,
-                     plist=plist, pscale=pscale, x0=x0, sx0=sx0, copy_inputs=True)
+                     plist=plist, pscale=pscale, x0=x0, sx0=sx0, t0=t0, copy_inputs=True)
         self.model = model
         self.inputs = p["inputs"]
         self.out = p["out"]
@@ -790,9 +813,10 @@ def run_simulations(model: Model, solver: Solver, edata_list: Sequence[ExpData],
           if has_x0 else None)
     sx0 = (np.stack([np.asarray(e.sx0, np.float64).reshape(len(plist), model.nx_rdata)
                      for e in edata_list]) if has_sx0 else None)
+    t0 = np.array([model.t0() if e.t_start is None else float(e.t_start) for e in edata_list])
     out = simulate_batch(model, solver, theta, kfix, ts, meas, sigma, want_x="x" in fields,
                          want_sx=fsa, kfix_preeq=kpre, plist=list(plist), pscale=list(pscale),
-                         x0=x0, sx0=sx0, failfast=failfast)
+                         x0=x0, sx0=sx0, t0=t0, failfast=failfast)
     dev = out["llh"].device
     meas_t, sigma_t = out["_keepalive"][3], out["_keepalive"][4]
     sy_t = None

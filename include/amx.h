@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define AMX_ABI_VERSION 9
+#define AMX_ABI_VERSION 10
 #define AMX_NSTATS 18
 #define AMX_NDIAG 9
 /* bytes of the static workspace of amx_run_batch_static, per GPU */
@@ -117,12 +117,17 @@ typedef struct {
      *   from it too.
      * sx0: [nb][nplist][nx] their sensitivities with respect to the scaled
      *   parameters (ExpData::sx0, Model::set_initial_state_sensitivities;
-     *   [0]); not together with kfix_preeq (-22).                            */
+     *   [0]); not together with kfix_preeq (-22).
+     * t0: [nb] start time of each instance's simulation (ExpData t_start,
+     *   Model::set_t0, applied in ConditionContext::apply_condition,
+     *   src/edata.cpp:551; [0]); output times at or before it give the initial
+     *   state, and the adjoint sweep ends there.                              */
     int nplist;
     const int* plist;
     const int* pscale;
     const double* x0;
     const double* sx0;
+    const double* t0;
 } amx_batch_in;
 
 typedef struct {

@@ -160,20 +160,21 @@ def test_teams_narrower_than_a_wavefront_are_refused():
         codegen.emit_model_traits(networks.network_n40(), 32)
 
 
-def test_abi_v9_struct_layout():
+def test_abi_v10_struct_layout():
     """amx_solver_opts ends with max_blocks, interp, stldet, failfast and
-    ckpt_steps, amx_batch_in with the parameter list, scales, initial states
-    and their sensitivities, amx_batch_out with diag; 18 counters per instance
-    (include/amx.h, ABI version 9)."""
-    assert _lib.AMX_ABI_VERSION == 9 and _lib.AMX_NSTATS == 18
-    assert [f[0] for f in _lib.BatchIn._fields_[-5:]] == ["nplist", "plist", "pscale", "x0", "sx0"]
+    ckpt_steps, amx_batch_in with the parameter list, scales, initial states,
+    their sensitivities and the start times, amx_batch_out with diag; 18
+    counters per instance (include/amx.h, ABI version 10)."""
+    assert _lib.AMX_ABI_VERSION == 10 and _lib.AMX_NSTATS == 18
+    assert [f[0] for f in _lib.BatchIn._fields_[-6:]] == ["nplist", "plist", "pscale", "x0", "sx0",
+                                                          "t0"]
     assert [f[0] for f in _lib.SolverOpts._fields_[-5:]] == ["max_blocks", "interp", "stldet",
                                                              "failfast", "ckpt_steps"]
     assert _lib.make_opts().failfast == 0 and _lib.make_opts(failfast=True).failfast == 1
     assert _lib.make_opts().ckpt_steps == 0 and _lib.make_opts(ckpt_steps=32).ckpt_steps == 32
     assert _lib.BatchOut._fields_[-1][0] == "diag"
     text = (ROOT / "include" / "amx.h").read_text()
-    assert "#define AMX_ABI_VERSION 9" in text and "#define AMX_NDIAG 9" in text
+    assert "#define AMX_ABI_VERSION 10" in text and "#define AMX_NDIAG 9" in text
     assert "#define AMX_NSTATS 18" in text
     assert "#define AMX_INTERP_HERMITE 1" in text and "#define AMX_INTERP_POLYNOMIAL 2" in text
     assert "#define AMX_STLDET_ON 1" in text and "#define AMX_STLDET_OFF 2" in text
@@ -310,3 +311,13 @@ def test_sx0_with_preequilibration_rejected(lib):
     o = _lib.make_opts(sensi_meth=_lib.SENSI_FORWARD)
     assert L.lib.amx_run_batch(ctypes.byref(o), ctypes.byref(ins), ctypes.byref(outs),
                                p, 8, None) == -22
+
+
+def test_model_start_time_setting():
+    from amici_amd import sim
+    m = sim.Model(networks.robertson())
+    assert m.t0() == 0.0
+    m.set_t0(2.5)
+    assert m.t0() == 2.5
+    e = sim.ExpData(m)
+    assert e.t_start is None
