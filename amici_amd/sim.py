@@ -409,6 +409,8 @@ class ExpData:
             self.fixed_parameters = np.zeros(0)
             self.free_parameters = np.zeros(0)
             self.fixed_parameters_pre_equilibration = np.zeros(0)
+            self.fixed_parameters_presimulation = np.zeros(0)
+            self.t_presim = 0.0
             self.plist, self.pscale, self.x0, self.sx0 = [], [], np.zeros(0), np.zeros(0)
             self.t_start = None
             nt = len(self.timepoints)
@@ -427,6 +429,10 @@ class ExpData:
         self.fixed_parameters = model.get_fixed_parameters() if model else np.zeros(0)
         self.free_parameters = np.zeros(0)
         self.fixed_parameters_pre_equilibration = np.zeros(0)
+        # pre-simulation over t_presim before the start time with these fixed
+        # parameters (empty: the condition's)
+        self.fixed_parameters_presimulation = np.zeros(0)
+        self.t_presim = 0.0
         self.plist, self.pscale, self.x0, self.sx0 = [], [], np.zeros(0), np.zeros(0)
         self.t_start = None  # start time (None: the model's t0)
         self._meas = np.full((len(self.timepoints), self.ny), np.nan)
@@ -791,8 +797,22 @@ def run_simulations(model: Model, solver: Solver, edata_list: Sequence[ExpData],
     _, plist, pscale, has_x0, has_sx0 = keys[0]
     theta = np.stack([e.free_parameters if len(e.free_parameters) else model.get_free_parameters()
                       for e in edata_list])
-    kfix = np.stack([np.asarray(e.fixed_parameters, np.float64) if len(e.fixed_parameters)
-                     else model.get_fixed_parameters() for e in edata_list]).reshape(nb, model.nk)
+    # Pre-simulation (ForwardProblem::handle_presimulation,
+    # src/forwardproblem.cpp:284-321): the simulation starts at t0 - t_presim with
+    # the pre-simulation's fixed parameters and continues through t0 without
+    # reinitialisation (the reference refuses reinitialised states there), and
+    # the adjoint sweep runs back through it (backwardproblem.cpp:53-64).  The
+    # fixed parameters of these models only set the initial amounts of input
+    # species (and the species reset after a pre-equilibration), so the
+    # pre-simulation is the condition run from t0 - t_presim with the
+    # pre-simulation's fixed parameters.
+    def kfix_of(e):
+        k = e.fixed_parameters
+        if float(getattr(e, "t_presim", 0.0)) > 0 and len(e.fixed_parameters_presimulation):
+            k = e.fixed_parameters_presimulation
+        return np.asarray(k, np.float64) if len(k) else model.get_fixed_parameters()
+
+    kfix = np.stack([kfix_of(e) for e in edata_list]).reshape(nb, model.nk)
     ts = np.stack([e.timepoints for e in edata_list])
     meas = np.stack([e.get_measurements() for e in edata_list])
     sigma = np.stack([e.get_noise_scales() for e in edata_list])
@@ -813,7 +833,8 @@ def run_simulations(model: Model, solver: Solver, edata_list: Sequence[ExpData],
           if has_x0 else None)
     sx0 = (np.stack([np.asarray(e.sx0, np.float64).reshape(len(plist), model.nx_rdata)
                      for e in edata_list]) if has_sx0 else None)
-    t0 = np.array([model.t0() if e.t_start is None else float(e.t_start) for e in edata_list])
+    t0 = np.array([(model.t0() if e.t_start is None else float(e.t_start))
+                   - max(float(getattr(e, "t_presim", 0.0)), 0.0) for e in edata_list])
     out = simulate_batch(model, solver, theta, kfix, ts, meas, sigma, want_x="x" in fields,
                          want_sx=fsa, kfix_preeq=kpre, plist=list(plist), pscale=list(pscale),
                          x0=x0, sx0=sx0, t0=t0, failfast=failfast)

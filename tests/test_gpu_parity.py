@@ -638,3 +638,32 @@ def test_start_time(models, model, fixture):
     for i, r in enumerate(rd):
         assert r.status == 0
         _close(np.asarray(r.sllh)[None], g["sllh_asa"][i:i + 1], 1e-5, 1e-7, "sllh (run_simulations)")
+
+
+def test_presimulation(models):
+    """Pre-simulation (ExpData t_presim, fixed_parameters_presimulation):
+    the condition runs from t0 - t_presim with the pre-simulation's fixed
+    parameters, which set the input species' initial amounts, and continues
+    through t0 without reinitialisation.  With t_presim two output intervals
+    of the N40 grid, the states at ts[k] are the oracle's states at ts[k + 2]
+    of the run with the pre-simulation inputs; the condition's own fixed
+    parameters (here deliberately different) play no part."""
+    from amici_amd import sim
+    g = _load("n40_conditions16")
+    m = models["n40"]
+    ts = g["ts"]
+    T = ts[2] - ts[0]
+    sol = sim.Solver(**TIGHT)
+    eds = []
+    for i in range(8):
+        e = sim.ExpData(m)
+        e.set_timepoints(ts[:-2])
+        e.free_parameters = g["theta"][i]
+        e.fixed_parameters = g["kfix"][(i + 1) % 8] * 3.0
+        e.fixed_parameters_presimulation = g["kfix"][i]
+        e.t_presim = T
+        eds.append(e)
+    rd = sim.run_simulations(m, sol, eds)
+    x = np.stack([r.x for r in rd])
+    assert all(r.status == 0 for r in rd)
+    _close(x, g["x"][:8, 2:], Y_RTOL, Y_FLOOR, "x after pre-simulation")
