@@ -37,7 +37,8 @@ typedef struct {
     const double* x0c; /* [nx] constant initial concentration           */
     const int* x0k;    /* [nx] -1, or index into k giving x0            */
     const double* C;   /* [ny*nx] row-major observable matrix y = C x   */
-    int noise;         /* 0: normal, 1: log-normal                       */
+    int noise;         /* 0 normal, 1 log-normal, 2 log10-normal, 3 laplace,
+                          4 log-laplace, 5 log10-laplace */
 } amx_oracle_model;
 
 typedef struct {
