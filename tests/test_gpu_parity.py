@@ -667,3 +667,28 @@ def test_presimulation(models):
     x = np.stack([r.x for r in rd])
     assert all(r.status == 0 for r in rd)
     _close(x, g["x"][:8, 2:], Y_RTOL, Y_FLOOR, "x after pre-simulation")
+
+
+def test_model_start_time_default(models):
+    """Model.set_t0 applies to every condition of a batch that gives no start
+    time of its own (ExpData t_start overrides it, src/edata.cpp:551)."""
+    from amici_amd import sim
+    g = _load("robertson_batch64")
+    m = sim.Model(networks.robertson())
+    m.set_t0(12.5)
+    shifted = dict(g, ts=g["ts"] + 12.5)
+    out = _run_kw(m, shifted, sim.SensitivityMethod.adjoint)
+    assert (out["status"] == 0).all(), out["status"]
+    _close(out["x"], g["x"], Y_RTOL, Y_FLOOR, "x")
+    _close(out["sllh"], g["sllh_asa"], 1e-5, 1e-7, "sllh (model t0)")
+    # a condition's own start time wins
+    e = sim.ExpData(m)
+    e.set_timepoints(g["ts"])
+    e.t_start = 0.0
+    e.free_parameters = g["theta"][0]
+    e.set_measurements(g["meas"][0])
+    e.set_noise_scales(g["sigma"][0])
+    sol = sim.Solver(**TIGHT)
+    r = sim.run_simulations(m, sol, [e])[0]
+    assert r.status == 0
+    _close(np.asarray(r.x)[None], g["x"][:1], Y_RTOL, Y_FLOOR, "x (ExpData t_start 0)")
