@@ -765,6 +765,30 @@ def simulate_concurrent(groups: Sequence[dict]) -> list:
     return outs
 
 
+def condition_start(model: Model, edata_list) -> tuple:
+    """Fixed parameters [nb, nk] and start times [nb] of the conditions as the
+    kernels run them.  The start time is ExpData t_start, else the model's t0
+    (ConditionContext::apply_condition, src/edata.cpp:551).  Pre-simulation
+    (ForwardProblem::handle_presimulation, src/forwardproblem.cpp:284-321)
+    starts t_presim earlier with the pre-simulation's fixed parameters and
+    continues through t0 without reinitialisation (the reference refuses
+    reinitialised states there), the adjoint sweep running back through it
+    (backwardproblem.cpp:53-64).  The fixed parameters of these models only
+    set the initial amounts of input species (and the species reset after a
+    pre-equilibration), so a pre-simulation is the condition run from
+    t0 - t_presim with the pre-simulation's fixed parameters."""
+    kfix, t0 = [], []
+    for e in edata_list:
+        pre = float(getattr(e, "t_presim", 0.0)) > 0
+        k = e.fixed_parameters
+        if pre and len(e.fixed_parameters_presimulation):
+            k = e.fixed_parameters_presimulation
+        kfix.append(np.asarray(k, np.float64) if len(k) else model.get_fixed_parameters())
+        start = model.t0() if e.t_start is None else float(e.t_start)
+        t0.append(start - (float(e.t_presim) if pre else 0.0))
+    return np.stack(kfix).reshape(len(edata_list), model.nk), np.array(t0)
+
+
 def run_simulations(model: Model, solver: Solver, edata_list: Sequence[ExpData],
                     failfast: bool = True, num_threads: int = 1) -> list:
     """All conditions of ``edata_list`` as one GPU batch; returns one
@@ -797,22 +821,7 @@ def run_simulations(model: Model, solver: Solver, edata_list: Sequence[ExpData],
     _, plist, pscale, has_x0, has_sx0 = keys[0]
     theta = np.stack([e.free_parameters if len(e.free_parameters) else model.get_free_parameters()
                       for e in edata_list])
-    # Pre-simulation (ForwardProblem::handle_presimulation,
-    # src/forwardproblem.cpp:284-321): the simulation starts at t0 - t_presim with
-    # the pre-simulation's fixed parameters and continues through t0 without
-    # reinitialisation (the reference refuses reinitialised states there), and
-    # the adjoint sweep runs back through it (backwardproblem.cpp:53-64).  The
-    # fixed parameters of these models only set the initial amounts of input
-    # species (and the species reset after a pre-equilibration), so the
-    # pre-simulation is the condition run from t0 - t_presim with the
-    # pre-simulation's fixed parameters.
-    def kfix_of(e):
-        k = e.fixed_parameters
-        if float(getattr(e, "t_presim", 0.0)) > 0 and len(e.fixed_parameters_presimulation):
-            k = e.fixed_parameters_presimulation
-        return np.asarray(k, np.float64) if len(k) else model.get_fixed_parameters()
-
-    kfix = np.stack([kfix_of(e) for e in edata_list]).reshape(nb, model.nk)
+    kfix, t0 = condition_start(model, edata_list)
     ts = np.stack([e.timepoints for e in edata_list])
     meas = np.stack([e.get_measurements() for e in edata_list])
     sigma = np.stack([e.get_noise_scales() for e in edata_list])
@@ -833,8 +842,6 @@ def run_simulations(model: Model, solver: Solver, edata_list: Sequence[ExpData],
           if has_x0 else None)
     sx0 = (np.stack([np.asarray(e.sx0, np.float64).reshape(len(plist), model.nx_rdata)
                      for e in edata_list]) if has_sx0 else None)
-    t0 = np.array([(model.t0() if e.t_start is None else float(e.t_start))
-                   - max(float(getattr(e, "t_presim", 0.0)), 0.0) for e in edata_list])
     out = simulate_batch(model, solver, theta, kfix, ts, meas, sigma, want_x="x" in fields,
                          want_sx=fsa, kfix_preeq=kpre, plist=list(plist), pscale=list(pscale),
                          x0=x0, sx0=sx0, t0=t0, failfast=failfast)

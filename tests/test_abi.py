@@ -321,3 +321,22 @@ def test_model_start_time_setting():
     assert m.t0() == 2.5
     e = sim.ExpData(m)
     assert e.t_start is None
+
+
+def test_condition_start_times_and_presimulation():
+    """run_simulations' start times and fixed parameters: ExpData t_start,
+    else the model's t0; a pre-simulation starts t_presim earlier with the
+    pre-simulation's fixed parameters (the condition's own when those are
+    empty)."""
+    from amici_amd import sim
+    m = sim.Model(networks.network_n40())
+    m.set_t0(1.5)
+    k = np.arange(m.nk, dtype=float) + 1.0
+    a, b, c, d = (sim.ExpData(m) for _ in range(4))
+    a.fixed_parameters = k
+    b.fixed_parameters, b.t_start = k, 4.0
+    c.fixed_parameters, c.t_presim, c.fixed_parameters_presimulation = k, 2.0, 10 * k
+    d.fixed_parameters, d.t_presim, d.t_start = k, 3.0, 0.0
+    kfix, t0 = sim.condition_start(m, [a, b, c, d])
+    np.testing.assert_array_equal(t0, [1.5, 4.0, -0.5, -3.0])
+    np.testing.assert_array_equal(kfix, np.stack([k, k, 10 * k, k]))
